@@ -1,0 +1,157 @@
+/*
+ * smx.h — C ABI of libsmx.so, the MI355X (gfx950) implementation of
+ * SchwarMAX's model-evaluation path.
+ *
+ * The reference has no FFI: its boundary is a set of JAX functions.  Each entry
+ * point below names the reference function whose computation it replaces.
+ * Conventions:
+ *   - every array argument is a caller-owned DEVICE pointer (torch tensors'
+ *     data_ptr() on the Python side), row-major, with the shape in its comment;
+ *   - scratch memory is passed in by the caller (sizes from the *_bytes queries);
+ *   - `stream` is a hipStream_t passed as void*; every call is asynchronous on it;
+ *   - the return value is 0 on success, non-zero on error; smx_last_error()
+ *     then returns a message.  Numerical failures are not errors: as in the
+ *     reference they become NaN, zero weights or invalid orbits.
+ */
+#ifndef SMX_H
+#define SMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Canonical potential NFW + Miyamoto-Nagai + T3 bar + V4 bulge and the
+ * canonical stellar density MN + T3 + V4 (likelihood.py:13-71).  Filled on the
+ * host from the reference's parameter dicts. */
+typedef struct smx_potential {
+  /* NFW halo (potentials.py:18-27) */
+  double nfw_mGM;        /* -G * 10**logM                                 */
+  double nfw_Rs, nfw_a, nfw_b, nfw_c;
+  double nfw_origin[3];
+  double nfw_rot[9];     /* get_mat(dirx, diry, dirz), row-major          */
+  /* Miyamoto-Nagai disc (potentials.py:29-65) */
+  double mn_mGM;         /* -G * 10**logM_disc                            */
+  double mn_rho0;        /* Hs_disc**2 * 10**logM_disc / (4 pi)           */
+  double mn_Rs, mn_Hs;
+  double mn_origin[3];
+  double mn_rot[9];
+  /* Dehnen & Aly T3 bar and V4 bulge (dehnen_bar.py:1318-1383, 1425-1543) */
+  double t3_M, t3_a, t3_b, t3_L, t3_gamma;
+  double v4_M, v4_a, v4_b, v4_L, v4_gamma;
+  double G;
+} smx_potential;
+
+/* Binning geometry (integrates.py:37-52, utils.py:10-42). */
+typedef struct smx_grids {
+  int32_t nR, nz, nphi;    /* (R, z, phi) cell grid, strides 1, nR, nR*nz   */
+  int32_t nX, nY;          /* sky pixel grid                                 */
+  int32_t num_vbin;        /* apertures B                                    */
+  int32_t symmetry;        /* 1: fourfold bar symmetry (integrates.py:74-79) */
+  int32_t pad_;
+  double rz_min[3], rz_max[3];
+  double xy_min[2], xy_max[2];
+  double rot[9];           /* makeRotationMatrix(alpha, beta, gamma)         */
+  double area_pixel;       /* pixel area in pc^2 (integrates.py:70-71)       */
+} smx_grids;
+
+/* Integration schedule (model.py:163-180). */
+typedef struct smx_schedule {
+  int32_t N_max;           /* BS3 attempts per integration (fixed)           */
+  int32_t pad_;
+  double Omega;            /* ODE pattern speed, = -Omega_bar (model.py:173) */
+  double atol, rtol, dt_min, dt_max;
+} smx_schedule;
+
+/* ---- library management ---------------------------------------------- */
+int smx_last_error(char* buf, int64_t len);
+int smx_version(void);
+
+/* ---- potential (potentials.py, dehnen_bar.py, likelihood.py:13-71) ------ */
+/* Phi and acc = -grad Phi at n points xyz (n,3); acc may be NULL.
+ * Replaces potential_func / acc_fn (likelihood.py:46-71, model.py:152-161). */
+int smx_potential_eval(const smx_potential* p, const double* xyz, int64_t n,
+                       double* phi, double* acc, void* stream);
+/* Stellar density at n points.  Replaces density_func (likelihood.py:13-43). */
+int smx_density_eval(const smx_potential* p, const double* xyz, int64_t n,
+                     double* rho, void* stream);
+
+/* ---- initial conditions (jeans.py:9-94, model.py:112-169) --------------- */
+/* Jeans velocities from positions w0 (n,3) and normals (n,3), rotation curve,
+ * jitter of r realizations from uniforms (n,r_total,6) (first r used), orbital
+ * time scale.  Outputs wb (n,r,6), T_total (n,r) = N_dyn*T_orb,
+ * dt0 (n,r) = T_orb/N_step.  Replaces get_w0_new + the jitter/timescale block
+ * of build_model (model.py:112-169). */
+int smx_initial_conditions(const smx_potential* p, const double* w0, const double* normals,
+                           const double* uniforms, int64_t n, int32_t r, int32_t r_total,
+                           double N_step_per_orb, double N_dyn,
+                           double* wb, double* T_total, double* dt0, void* stream);
+
+/* ---- orbit library (integrates.py:37-313) ------------------------------- */
+/* Bytes of the per-integration accumulator workspace for n_int integrations. */
+int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int);
+/* Integrate n_int = n*r orbits with fused binning into the workspace, then
+ * average realizations into lib (n, P + 6B) laid out as
+ * [Rzphi (P) | surface density (B) | h1 | h2 | h3 | h4 | h0] per column.
+ * valid/T_int/t_final (n_int,) are per-integration diagnostics.
+ * Replaces _integrate_adaptive_batch_chunked_vmap (integrates.py:303-313). */
+int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                      const double* wb, const double* T_total, const double* dt0,
+                      int64_t n, int32_t r, const int32_t* bin_mapping,
+                      const double* num_per_bin, const double* v0, const double* sdisp,
+                      void* workspace, double* lib, double* valid, double* T_int,
+                      double* t_final, void* stream);
+/* Trajectory variant, no symmetry, dt normalised (integrates.py:319-581):
+ * o_traj (n_int, N_max, 6) with velocities in km/s, t_traj (n_int, N_max),
+ * per-integration library rows lib_int (n_int, P + 6B) and the realization
+ * average lib (n, P + 6B). */
+int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                           const double* wb, const double* T_total, const double* dt0,
+                           int64_t n, int32_t r, const int32_t* bin_mapping,
+                           const double* num_per_bin, const double* v0, const double* sdisp,
+                           void* workspace, double* lib, double* o_traj, double* t_traj,
+                           double* valid, double* T_int, void* stream);
+/* Bin externally supplied samples (S,6) with weights dt (S,) into ONE
+ * histogram: the bit-exactness hook for the grid rules (utils.py:10-42,
+ * integrates.py:115-174).  Writes per-sample cell / aperture ids for each
+ * symmetry copy (S, nsym) (-1 when dropped), and the sums acc_rz (P) and
+ * acc_ap (B, 6) = [dt, g dt H0 .. g dt H4]. */
+int smx_bin_samples(const smx_grids* g, const double* samples, const double* dt, int64_t S,
+                    const int32_t* bin_mapping, const double* v0, const double* sdisp,
+                    int32_t* idx_rz, int32_t* idx_ap, double* acc_rz, double* acc_ap,
+                    void* stream);
+
+/* ---- density target (model.py:192-213) ---------------------------------- */
+/* Cell masses by Sobol quadrature: out (P) from cell centres (P) x3 and
+ * samples (S,3). */
+int smx_density_target(const smx_potential* p, const double* R_grid, const double* z_grid,
+                       const double* phi_grid, int64_t P, double dR, double dz, double dphi,
+                       const double* sample, int64_t S, double* out, void* stream);
+
+/* ---- NNLS (nnls.py:6-166) ----------------------------------------------- */
+/* Bytes of NNLS workspace for n orbits, m rows, K right-hand sides. */
+int64_t smx_nnls_workspace_bytes(int64_t n, int64_t m, int64_t K);
+/* Bootstrap ADMM NNLS.  lib (n, ld_lib) holds the library columns whose first
+ * m entries are the design rows; row_scale (m) turns them into U
+ * (U[j,i] = lib[i,j] * row_scale[j], FP32).  Y (K, m) are the stacked data
+ * vectors.  Solves K problems with the shared factor of U^T U + lambda/n I,
+ * maxiter ADMM steps (alpha = 1.6, rho = tr(Q)/n, start w = z = w_init).
+ * Writes W (K, n).  tol > 0 stops each right-hand side early once an
+ * iterate changes by less than tol (relative, max norm).
+ * Replaces solve_nnls_admm_bootstrap (nnls.py:68-166) and, with K = 1,
+ * solve_nnls_admm (nnls.py:6-66). */
+int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
+                  const double* row_scale, const double* Y, int64_t K,
+                  double lambda_reg, int32_t maxiter, double w_init, double tol,
+                  void* workspace, double* W, int32_t* iters_done, void* stream);
+
+/* ---- model maps and likelihood (model.py:14-68, likelihood.py:199-244) -- */
+/* maps (K, m) = W (K, n) . lib[:, :m]^T in FP64. */
+int smx_model_maps(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
+                   const double* W, int64_t K, double* maps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMX_H */

@@ -1,0 +1,139 @@
+"""ctypes binding of libsmx.so (the C ABI declared in include/smx.h).
+
+Device memory is owned by torch tensors; every call passes raw device pointers
+and the current HIP stream.  There is no CPU fallback: loading fails loudly when
+the library or a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libsmx.so')
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class SmxPotential(ctypes.Structure):
+    _fields_ = [
+        ('nfw_mGM', ctypes.c_double), ('nfw_Rs', ctypes.c_double), ('nfw_a', ctypes.c_double),
+        ('nfw_b', ctypes.c_double), ('nfw_c', ctypes.c_double),
+        ('nfw_origin', ctypes.c_double * 3), ('nfw_rot', ctypes.c_double * 9),
+        ('mn_mGM', ctypes.c_double), ('mn_rho0', ctypes.c_double), ('mn_Rs', ctypes.c_double),
+        ('mn_Hs', ctypes.c_double), ('mn_origin', ctypes.c_double * 3), ('mn_rot', ctypes.c_double * 9),
+        ('t3_M', ctypes.c_double), ('t3_a', ctypes.c_double), ('t3_b', ctypes.c_double),
+        ('t3_L', ctypes.c_double), ('t3_gamma', ctypes.c_double),
+        ('v4_M', ctypes.c_double), ('v4_a', ctypes.c_double), ('v4_b', ctypes.c_double),
+        ('v4_L', ctypes.c_double), ('v4_gamma', ctypes.c_double),
+        ('G', ctypes.c_double),
+    ]
+
+
+class SmxGrids(ctypes.Structure):
+    _fields_ = [
+        ('nR', ctypes.c_int32), ('nz', ctypes.c_int32), ('nphi', ctypes.c_int32),
+        ('nX', ctypes.c_int32), ('nY', ctypes.c_int32), ('num_vbin', ctypes.c_int32),
+        ('symmetry', ctypes.c_int32), ('pad_', ctypes.c_int32),
+        ('rz_min', ctypes.c_double * 3), ('rz_max', ctypes.c_double * 3),
+        ('xy_min', ctypes.c_double * 2), ('xy_max', ctypes.c_double * 2),
+        ('rot', ctypes.c_double * 9), ('area_pixel', ctypes.c_double),
+    ]
+
+
+class SmxSchedule(ctypes.Structure):
+    _fields_ = [
+        ('N_max', ctypes.c_int32), ('pad_', ctypes.c_int32), ('Omega', ctypes.c_double),
+        ('atol', ctypes.c_double), ('rtol', ctypes.c_double), ('dt_min', ctypes.c_double),
+        ('dt_max', ctypes.c_double),
+    ]
+
+
+# (name, restype, argtypes) of every entry point in include/smx.h
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+_SIGS = [
+    ('smx_last_error', ctypes.c_int, [ctypes.c_char_p, _I64]),
+    ('smx_version', ctypes.c_int, []),
+    ('smx_potential_eval', ctypes.c_int, [ctypes.POINTER(SmxPotential), _P, _I64, _P, _P, _P]),
+    ('smx_density_eval', ctypes.c_int, [ctypes.POINTER(SmxPotential), _P, _I64, _P, _P]),
+    ('smx_initial_conditions', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, _I32, _I32, _D, _D, _P, _P, _P, _P]),
+    ('smx_library_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I64]),
+    ('smx_build_library', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_build_library_traj', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_bin_samples', ctypes.c_int,
+     [ctypes.POINTER(SmxGrids), _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_density_target', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, _D, _D, _D, _P, _I64, _P, _P]),
+    ('smx_nnls_workspace_bytes', _I64, [_I64, _I64, _I64]),
+    ('smx_nnls_admm', ctypes.c_int,
+     [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
+    ('smx_model_maps', ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _P, _P]),
+]
+
+EXPORTED = [s[0] for s in _SIGS]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libsmx.so and bind signatures (does not touch the GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsmx.so not built at {path}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, res, args in _SIGS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib():
+    """The loaded library, after checking a GPU is present (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("schwarmax_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return load_library()
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(2048)
+    load_library().smx_last_error(buf, 2048)
+    return buf.value.decode(errors='replace')
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (code {rc}): {last_error()}")
+
+
+def ptr(t):
+    """Device pointer of a contiguous tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def dev(a, dtype=torch.float64, device='cuda'):
+    """Host or device array -> contiguous device tensor of dtype."""
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)

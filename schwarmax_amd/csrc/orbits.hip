@@ -1,0 +1,616 @@
+// Orbit library kernels: potential evaluation, Jeans initial conditions,
+// fused BS3 integration + binning, realization averaging, trajectory path,
+// the binning test hook and the density target.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <cstdarg>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+#include "potential.hpp"
+#include "binning.hpp"
+
+namespace smx {
+
+// ---------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------
+static std::mutex g_err_mu;
+static std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err = buf;
+}
+
+// ---------------------------------------------------------------------------
+// potential / density at points
+// ---------------------------------------------------------------------------
+__global__ void potential_eval_kernel(smx_potential p, const double* __restrict__ xyz, int64_t n,
+                                      double* __restrict__ phi, double* __restrict__ acc) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  if (acc) {
+    double ax, ay, az;
+    const double ph = accel(p, x, y, z, ax, ay, az);
+    acc[3 * i] = ax;
+    acc[3 * i + 1] = ay;
+    acc[3 * i + 2] = az;
+    if (phi) phi[i] = ph;
+  } else if (phi) {
+    phi[i] = phi_total<double>(p, x, y, z);
+  }
+}
+
+__global__ void density_eval_kernel(smx_potential p, const double* __restrict__ xyz, int64_t n,
+                                    double* __restrict__ rho) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rho[i] = rho_total(p, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+}
+
+// ---------------------------------------------------------------------------
+// Jeans initial conditions (jeans.py:9-94) + jitter / time scales
+// (model.py:117-169).  One 64-lane wavefront per star.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// trapezoid weights for numpy-style linspace(lo, 10, npts): point i, interval
+// i -> i+1 has width pts[i+1]-pts[i]
+__device__ __forceinline__ double lin_pt(double lo, double hi, double step, int i, int npts) {
+  return (i == npts - 1) ? hi : (static_cast<double>(i) * step + lo);
+}
+
+__device__ __forceinline__ double dphi_dz(const smx_potential& p, double x, double y, double z) {
+  const double d = 5e-3;
+  return (phi_total<double>(p, x, y, z + d) - phi_total<double>(p, x, y, z - d)) / (2.0 * d);
+}
+
+// integral_{|z*|}^{10} rho(x, y, z') dPhi/dz(x, y, z') dz' by the trapezoid
+// rule on npts points, this lane's share; summed over the wave by the caller.
+__device__ double pressure_part(const smx_potential& p, double x, double y, double zlo, int npts,
+                                int lane) {
+  const double hi = 10.0;
+  const double step = (hi - zlo) / static_cast<double>(npts - 1);
+  double acc = 0.0;
+  for (int i = lane; i < npts; i += 64) {
+    const double zi = lin_pt(zlo, hi, step, i, npts);
+    const double f = rho_total(p, x, y, zi) * dphi_dz(p, x, y, zi);
+    double w = 0.0;
+    if (i > 0) w += zi - lin_pt(zlo, hi, step, i - 1, npts);
+    if (i < npts - 1) w += lin_pt(zlo, hi, step, i + 1, npts) - zi;
+    acc += 0.5 * w * f;
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void initial_conditions_kernel(
+    smx_potential p, const double* __restrict__ w0, const double* __restrict__ normals,
+    const double* __restrict__ uniforms, int64_t n, int r, int r_total, double N_step,
+    double N_dyn, double* __restrict__ wb, double* __restrict__ T_total, double* __restrict__ dt0) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;   // whole wave exits together
+  const double xs = w0[3 * i], ys = w0[3 * i + 1], zs = w0[3 * i + 2];
+  const double Rs = sqrt(xs * xs + ys * ys);
+  const double zlo = fabs(zs);
+  const int npts = 1000;
+  const double dR = 5e-3;
+  const double I0 = wave_sum(pressure_part(p, xs, ys, zlo, npts, lane));
+  const double Ip = wave_sum(pressure_part(p, Rs + dR, 0.0, zlo, npts, lane));
+  const double Im = wave_sum(pressure_part(p, Rs - dR, 0.0, zlo, npts, lane));
+  if (lane != 0) return;
+  const double b = 1.0;   // anisotropy_b (jeans.py:86)
+  const double nu = rho_total(p, xs, ys, zs);
+  const double sz2 = nmax((1.0 / nu) * I0, 0.0);
+  const double sz = sqrt(sz2);
+  const double sR2 = b * sz2;
+  const double sR = sqrt(sR2);
+  const double dnu = b * (Ip - Im) / (2.0 * dR);
+  const double d = 5e-3;
+  const double dphidR = (phi_total<double>(p, Rs + d, 0.0, zs) - phi_total<double>(p, Rs - d, 0.0, zs)) / (2.0 * d);
+  const double vtot2 = (sR2 + (Rs / nu) * dnu) + Rs * dphidR;
+  const double sphi = sR;
+  double vmean = sqrt(nmax(vtot2 - sphi * sphi, 0.0));
+  double vR_s = sR, vz_s = sz, vp_s = sphi;
+  if (nu <= 0.0) { vmean = 0.0; vR_s = 0.0; vz_s = 0.0; vp_s = 0.0; }
+  const double g1 = normals[3 * i], g2 = normals[3 * i + 1], g3 = normals[3 * i + 2];
+  const double vR = g1 * vR_s, vz = g2 * vz_s, vphi = vmean + g3 * vp_s;
+  // clockwise cylindrical -> Cartesian (utils.py:144-162)
+  const double phi = atan2(ys, xs);
+  const double c = cos(phi), s = sin(phi);
+  const double wn[6] = {Rs * c, Rs * s, zs, vR * c + vphi * s, vR * s - vphi * c, vz};
+  // rotation curve and orbital time scale at the new ICs (utils.py:182-273)
+  const double R = sqrt(wn[0] * wn[0] + wn[1] * wn[1]);
+  const double dRt = 1e-3;
+  const double Rsafe = fmax(fabs(R), 2.0 * dRt);
+  const double dd = (phi_total<double>(p, Rsafe + dRt, 0.0, zs) - phi_total<double>(p, Rsafe - dRt, 0.0, zs)) / (2.0 * dRt);
+  const double vc = sqrt(nmax(Rsafe * dd, 0.0));
+  const double om = sqrt(nmax(fabs(dd) / Rsafe, 1e-20));
+  const double T_orb = (2.0 * kPi) / om;
+  const double vsc = nmin(nmax(0.1 * vc, 1.0), 15.0);
+  const double scale[6] = {0.1, 0.1, 0.1, vsc, vsc, vsc};
+  for (int k = 0; k < r; ++k) {
+    const int64_t o = i * r + k;
+    for (int j = 0; j < 6; ++j)
+      wb[6 * o + j] = wn[j] + (uniforms[(i * r_total + k) * 6 + j] - 0.5) * scale[j];
+    T_total[o] = T_orb * N_dyn;
+    dt0[o] = T_orb / N_step;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BS3 integrator (integrates.py:8-112)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void deriv(const smx_potential& p, double Om, const double* y, double* k) {
+  double ax, ay, az;
+  accel(p, y[0], y[1], y[2], ax, ay, az);
+  k[0] = y[3] + Om * y[1];
+  k[1] = y[4] - Om * y[0];
+  k[2] = y[5];
+  k[3] = ax + Om * y[4];
+  k[4] = ay - Om * y[3];
+  k[5] = az;
+}
+
+__device__ __forceinline__ double jacobi(const smx_potential& p, double Om, const double* y) {
+  const double ek = 0.5 * ((y[3] * y[3] + y[4] * y[4]) + y[5] * y[5]);
+  return (ek + phi_total<double>(p, y[0], y[1], y[2])) - Om * (y[0] * y[4] - y[1] * y[3]);
+}
+
+// One attempt.  On return y holds y_out, k1 the FSAL slope, dt the next step;
+// returns the step length used when accepted, 0 otherwise.
+__device__ __forceinline__ double bs3_attempt(const smx_potential& p, const smx_schedule& s,
+                                              double& t, double* y, double* k1, double& dt) {
+  const double Om = s.Omega;
+  double yt[6], k2[6], k3[6], k4[6], yn[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.5 * dt) * k1[j];
+  deriv(p, Om, yt, k2);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.75 * dt) * k2[j];
+  deriv(p, Om, yt, k3);
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    yn[j] = y[j] + dt * ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * k3[j]);
+  deriv(p, Om, yn, k4);
+  double es = 0.0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double e = dt * (((-5.0 / 72.0 * k1[j] + 1.0 / 12.0 * k2[j]) + 1.0 / 9.0 * k3[j]) +
+                           -1.0 / 8.0 * k4[j]);
+    const double sc = s.atol + s.rtol * nmax(fabs(y[j]), fabs(yn[j]));
+    const double q = e / sc;
+    es += q * q;
+  }
+  const double err = sqrt(es / 6.0);
+  const bool accept = err <= 1.0;
+  const double used = accept ? dt : 0.0;
+  if (accept) {
+    t = t + dt;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      y[j] = yn[j];
+      k1[j] = k4[j];
+    }
+  }
+  double sf = 0.9 * pow(nmax(err, 1e-10), -1.0 / 3.0);
+  sf = nmin(nmax(sf, 0.2), 5.0);
+  dt = nmin(nmax(dt * sf, s.dt_min), s.dt_max);
+  return used;
+}
+
+// per-integration accumulator slab: P cells (padded to 8) then B records of
+// 8 doubles [dt, g dt H0, ..., g dt H4, -, -]
+__host__ __device__ __forceinline__ int64_t slab_rz_len(int P) { return (P + 7) & ~7; }
+__host__ __device__ __forceinline__ int64_t slab_stride(int P, int B) {
+  return slab_rz_len(P) + 8 * (int64_t)B;
+}
+
+__device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __restrict__ bm,
+                                           const double* __restrict__ v0,
+                                           const double* __restrict__ sd, const double* y,
+                                           double w, int nsym, double* __restrict__ rz,
+                                           double* __restrict__ ap, int P) {
+  SampleProj sp;
+  project(g, y, nsym > 2, sp);
+  for (int c = 0; c < nsym; ++c) {
+    double zs, phi, xr, yr, vlos;
+    copy_coords(sp, c, zs, phi, xr, yr, vlos);
+    const int cell = rzphi_cell(g, sp.R, zs * y[2], phi);
+    if (cell >= 0) rz[cell] += w;
+    const int a = aperture(g, bm, xr, yr);
+    if (a >= 0) {
+      double gw, H[5];
+      gh_terms(vlos, v0[a], sd[a], gw, H);
+      double* rec = ap + 8 * (int64_t)a;
+      const double gd = gw * w;
+      rec[0] += w;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) rec[1 + k] += gd * H[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void integrate_library_kernel(
+    smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
+    const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
+    const int32_t* __restrict__ bm, const double* __restrict__ v0, const double* __restrict__ sd,
+    double* __restrict__ ws, double* __restrict__ valid_out, double* __restrict__ T_out,
+    double* __restrict__ tfin_out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n_int) return;
+  const int P = g.nR * g.nz * g.nphi;
+  const int64_t stride = slab_stride(P, g.num_vbin);
+  double* rz = ws + i * stride;
+  double* ap = rz + slab_rz_len(P);
+  double y[6], k1[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) y[j] = wb[6 * i + j];
+  deriv(p, s.Omega, y, k1);
+  const double EJ0 = jacobi(p, s.Omega, y);
+  double t = 0.0, dt = dt0[i], T_acc = 0.0;
+  const int nsym = g.symmetry ? 4 : 1;
+  for (int step = 0; step < s.N_max; ++step) {
+    const double used = bs3_attempt(p, s, t, y, k1, dt);
+    if (used != 0.0) {
+      bin_sample(g, bm, v0, sd, y, used, nsym, rz, ap, P);
+      T_acc += nsym * used;
+    }
+  }
+  const double EJ = jacobi(p, s.Omega, y);
+  const bool ok = (fabs(EJ / EJ0 - 1.0) < 0.1) && (t > T_total[i] / 10.0);
+  valid_out[i] = ok ? 1.0 : 0.0;
+  T_out[i] = T_acc + kEps;
+  tfin_out[i] = t;
+}
+
+// lib[c, :] = sum_r valid_r * (slab_r / T_r) / (sum_r valid_r + 0.1)
+// (integrates.py:212-236, 287-300); output row order
+// [Rzphi | surface density | h1 | h2 | h3 | h4 | h0].
+__global__ void finalize_library_kernel(const double* __restrict__ ws, int64_t n, int r, int P,
+                                        int B, const double* __restrict__ valid,
+                                        const double* __restrict__ T_int,
+                                        const double* __restrict__ num_per_bin, double area_pixel,
+                                        int pre_normalised, double* __restrict__ lib) {
+  const int64_t rows = (int64_t)P + 6 * (int64_t)B;
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n * rows) return;
+  const int64_t c = idx / rows;
+  const int64_t j = idx - c * rows;
+  const int64_t stride = slab_stride(P, B);
+  double nv = 0.0;
+  for (int k = 0; k < r; ++k) nv += valid[c * r + k];
+  const double wt = 1.0 / (nv + 0.1);
+  double out = 0.0;
+  for (int k = 0; k < r; ++k) {
+    const int64_t it = c * r + k;
+    const double* rz = ws + it * stride;
+    const double* ap = rz + slab_rz_len(P);
+    const double T = pre_normalised ? 1.0 : T_int[it];
+    double v;
+    if (j < P) {
+      v = rz[j] / T;
+    } else {
+      const int64_t q = j - P;
+      const int64_t blk = q / B, a = q - blk * B;
+      // blocks: 0 counts (surface density), 1..4 -> h1..h4, 5 -> h0
+      const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
+      v = ap[8 * a + comp] / T;
+      if (blk == 0) v = v / (num_per_bin[a] * area_pixel + kEps);
+    }
+    out += (valid[it] > 0.5 ? v : 0.0) * wt;
+  }
+  lib[idx] = out;
+}
+
+// Trajectory path (integrates.py:319-512): integrate storing every attempt's
+// state and dt; binning happens after, with dt / T.
+__global__ __launch_bounds__(64) void integrate_traj_kernel(
+    smx_potential p, smx_schedule s, const double* __restrict__ wb,
+    const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
+    double* __restrict__ traj, double* __restrict__ t_traj, double* __restrict__ valid_out,
+    double* __restrict__ T_out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n_int) return;
+  double y[6], k1[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) y[j] = wb[6 * i + j];
+  deriv(p, s.Omega, y, k1);
+  const double EJ0 = jacobi(p, s.Omega, y);
+  double t = 0.0, dt = dt0[i], csum = 0.0;
+  double* tr = traj + i * (int64_t)s.N_max * 6;
+  double* tt = t_traj + i * (int64_t)s.N_max;
+  for (int step = 0; step < s.N_max; ++step) {
+    const double used = bs3_attempt(p, s, t, y, k1, dt);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) tr[6 * step + j] = y[j];
+    csum += used;
+    tt[step] = used;     // dt_traj; turned into its cumulative sum by bin_traj_kernel
+  }
+  const double EJ = jacobi(p, s.Omega, y);
+  const bool ok = (fabs(EJ / EJ0 - 1.0) < 0.1) && (t > T_total[i] / 10.0);
+  valid_out[i] = ok ? 1.0 : 0.0;
+  T_out[i] = csum + kEps;
+}
+
+// Post-scan binning of stored trajectories, no symmetry, weights dt / T
+// (integrates.py:438-501); converts velocities to km/s afterwards (426-429).
+__global__ __launch_bounds__(64) void bin_traj_kernel(
+    smx_grids g, int64_t n_int, int N, double* __restrict__ traj, double* __restrict__ t_traj,
+    const double* __restrict__ T_int, const int32_t* __restrict__ bm, const double* __restrict__ v0,
+    const double* __restrict__ sd, double* __restrict__ ws) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n_int) return;
+  const int P = g.nR * g.nz * g.nphi;
+  double* rz = ws + i * slab_stride(P, g.num_vbin);
+  double* ap = rz + slab_rz_len(P);
+  double* tr = traj + i * (int64_t)N * 6;
+  double* tt = t_traj + i * (int64_t)N;
+  const double T = T_int[i];
+  double csum = 0.0;
+  for (int step = 0; step < N; ++step) {
+    const double used = tt[step];
+    csum += used;
+    tt[step] = csum;
+    double y[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) y[j] = tr[6 * step + j];
+    if (used != 0.0) {
+      SampleProj sp;
+      project(g, y, false, sp);
+      double zs, phi, xr, yr, vlos;
+      copy_coords(sp, 0, zs, phi, xr, yr, vlos);
+      const double w = used / T;
+      const int cell = rzphi_cell(g, sp.R, y[2], phi);
+      if (cell >= 0) rz[cell] += w;
+      const int a = aperture(g, bm, xr, yr);
+      if (a >= 0) {
+        double gw, H[5];
+        gh_terms(vlos, v0[a], sd[a], gw, H);
+        double* rec = ap + 8 * (int64_t)a;
+        rec[0] += w;
+        const double wg = w * gw;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rec[1 + k] += wg * H[k];
+      }
+    }
+#pragma unroll
+    for (int j = 3; j < 6; ++j) tr[6 * step + j] = y[j] * kKpcGyrToKms;
+  }
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Bit-exactness hook: one histogram over externally supplied samples.
+__global__ void bin_samples_kernel(smx_grids g, const double* __restrict__ samples,
+                                   const double* __restrict__ dt, int64_t S,
+                                   const int32_t* __restrict__ bm, const double* __restrict__ v0,
+                                   const double* __restrict__ sd, int32_t* __restrict__ idx_rz,
+                                   int32_t* __restrict__ idx_ap, double* __restrict__ acc_rz,
+                                   double* __restrict__ acc_ap) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const int nsym = g.symmetry ? 4 : 1;
+  double y[6];
+  for (int j = 0; j < 6; ++j) y[j] = samples[6 * i + j];
+  SampleProj sp;
+  project(g, y, nsym > 2, sp);
+  const double w = dt[i];
+  for (int c = 0; c < nsym; ++c) {
+    double zs, phi, xr, yr, vlos;
+    copy_coords(sp, c, zs, phi, xr, yr, vlos);
+    const int cell = rzphi_cell(g, sp.R, zs * y[2], phi);
+    const int a = aperture(g, bm, xr, yr);
+    idx_rz[i * nsym + c] = cell;
+    idx_ap[i * nsym + c] = a;
+    if (cell >= 0) atomicAdd(acc_rz + cell, w);
+    if (a >= 0) {
+      double gw, H[5];
+      gh_terms(vlos, v0[a], sd[a], gw, H);
+      atomicAdd(acc_ap + 6 * (int64_t)a, w);
+      const double gd = gw * w;
+      for (int k = 0; k < 5; ++k) atomicAdd(acc_ap + 6 * (int64_t)a + 1 + k, gd * H[k]);
+    }
+  }
+}
+
+// Cell masses by quasi-Monte-Carlo (model.py:192-213): one block per cell.
+__global__ __launch_bounds__(256) void density_target_kernel(
+    smx_potential p, const double* __restrict__ Rg, const double* __restrict__ zg,
+    const double* __restrict__ pg, int64_t P, double dR, double dz, double dphi,
+    const double* __restrict__ smp, int64_t S, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int64_t c = blockIdx.x;
+  if (c >= P) return;
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
+    const double R = Rg[c] + (smp[3 * k] - 0.5) * dR;
+    const double z = zg[c] + (smp[3 * k + 1] - 0.5) * dz;
+    const double ph = pg[c] + (smp[3 * k + 2] - 0.5) * dphi;
+    acc += rho_total(p, R * cos(ph), R * sin(ph), z) * R;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+    out[c] = ((tot / static_cast<double>(S)) * dR * dz) * dphi;
+  }
+}
+
+}  // namespace smx
+
+using namespace smx;
+
+extern "C" {
+
+int smx_last_error(char* buf, int64_t len) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  if (buf && len > 0) {
+    std::strncpy(buf, g_err.c_str(), static_cast<size_t>(len - 1));
+    buf[len - 1] = 0;
+  }
+  return static_cast<int>(g_err.size());
+}
+
+int smx_version(void) { return 1; }
+
+int smx_potential_eval(const smx_potential* p, const double* xyz, int64_t n, double* phi,
+                       double* acc, void* stream) {
+  SMX_REQUIRE(p && xyz && n >= 0, "smx_potential_eval: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(potential_eval_kernel, dim3(blocks_for(n, 128)), dim3(128), 0,
+                     as_stream(stream), *p, xyz, n, phi, acc);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_density_eval(const smx_potential* p, const double* xyz, int64_t n, double* rho,
+                     void* stream) {
+  SMX_REQUIRE(p && xyz && rho && n >= 0, "smx_density_eval: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(density_eval_kernel, dim3(blocks_for(n, 128)), dim3(128), 0,
+                     as_stream(stream), *p, xyz, n, rho);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_initial_conditions(const smx_potential* p, const double* w0, const double* normals,
+                           const double* uniforms, int64_t n, int32_t r, int32_t r_total,
+                           double N_step_per_orb, double N_dyn, double* wb, double* T_total,
+                           double* dt0, void* stream) {
+  SMX_REQUIRE(p && w0 && normals && uniforms && wb && T_total && dt0,
+              "smx_initial_conditions: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1 && r <= r_total, "smx_initial_conditions: bad sizes");
+  if (n == 0) return 0;
+  const int waves_per_block = 4;
+  hipLaunchKernelGGL(initial_conditions_kernel, dim3(blocks_for(n, waves_per_block)),
+                     dim3(64 * waves_per_block), 0, as_stream(stream), *p, w0, normals, uniforms,
+                     n, r, r_total, N_step_per_orb, N_dyn, wb, T_total, dt0);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int) {
+  if (!g) return -1;
+  return slab_stride(g->nR * g->nz * g->nphi, g->num_vbin) * n_int * (int64_t)sizeof(double);
+}
+
+static int check_grids(const smx_grids* g) {
+  SMX_REQUIRE(g->nR > 0 && g->nz > 0 && g->nphi > 0 && g->nX > 0 && g->nY > 0 && g->num_vbin > 0,
+              "grid sizes must be positive");
+  return 0;
+}
+
+int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                      const double* wb, const double* T_total, const double* dt0, int64_t n,
+                      int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
+                      const double* v0, const double* sdisp, void* workspace, double* lib,
+                      double* valid, double* T_int, double* t_final, void* stream) {
+  SMX_REQUIRE(p && g && s && wb && T_total && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
+                  workspace && lib && valid && T_int && t_final,
+              "smx_build_library: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0, "smx_build_library: bad sizes");
+  if (int e = check_grids(g)) return e;
+  if (n == 0) return 0;
+  const int64_t n_int = n * r;
+  const int P = g->nR * g->nz * g->nphi;
+  const int64_t words = slab_stride(P, g->num_vbin) * n_int;
+  hipStream_t st = as_stream(stream);
+  double* ws = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, ws, words, 0.0);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(integrate_library_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p,
+                     *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp, ws, valid, T_int,
+                     t_final);
+  SMX_LAUNCHED();
+  const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
+  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * rows, 256)), dim3(256), 0, st,
+                     ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 0, lib);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                           const double* wb, const double* T_total, const double* dt0, int64_t n,
+                           int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
+                           const double* v0, const double* sdisp, void* workspace, double* lib,
+                           double* o_traj, double* t_traj, double* valid, double* T_int,
+                           void* stream) {
+  SMX_REQUIRE(p && g && s && wb && T_total && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
+                  workspace && lib && o_traj && t_traj && valid && T_int,
+              "smx_build_library_traj: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0, "smx_build_library_traj: bad sizes");
+  SMX_REQUIRE(g->symmetry == 0, "smx_build_library_traj: the trajectory path has no symmetry");
+  if (int e = check_grids(g)) return e;
+  if (n == 0) return 0;
+  const int64_t n_int = n * r;
+  const int P = g->nR * g->nz * g->nphi;
+  const int64_t words = slab_stride(P, g->num_vbin) * n_int;
+  hipStream_t st = as_stream(stream);
+  double* ws = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, ws, words, 0.0);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(integrate_traj_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p, *s,
+                     wb, T_total, dt0, n_int, o_traj, t_traj, valid, T_int);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(bin_traj_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *g, n_int,
+                     s->N_max, o_traj, t_traj, T_int, bin_mapping, v0, sdisp, ws);
+  SMX_LAUNCHED();
+  const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
+  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * rows, 256)), dim3(256), 0, st,
+                     ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 1, lib);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_bin_samples(const smx_grids* g, const double* samples, const double* dt, int64_t S,
+                    const int32_t* bin_mapping, const double* v0, const double* sdisp,
+                    int32_t* idx_rz, int32_t* idx_ap, double* acc_rz, double* acc_ap,
+                    void* stream) {
+  SMX_REQUIRE(g && samples && dt && bin_mapping && v0 && sdisp && idx_rz && idx_ap && acc_rz &&
+                  acc_ap,
+              "smx_bin_samples: null pointer");
+  if (int e = check_grids(g)) return e;
+  if (S <= 0) return 0;
+  const int P = g->nR * g->nz * g->nphi;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(fill_kernel, dim3(64), dim3(256), 0, st, acc_rz, (int64_t)P, 0.0);
+  hipLaunchKernelGGL(fill_kernel, dim3(64), dim3(256), 0, st, acc_ap, 6 * (int64_t)g->num_vbin, 0.0);
+  hipLaunchKernelGGL(bin_samples_kernel, dim3(blocks_for(S, 128)), dim3(128), 0, st, *g, samples,
+                     dt, S, bin_mapping, v0, sdisp, idx_rz, idx_ap, acc_rz, acc_ap);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_density_target(const smx_potential* p, const double* R_grid, const double* z_grid,
+                       const double* phi_grid, int64_t P, double dR, double dz, double dphi,
+                       const double* sample, int64_t S, double* out, void* stream) {
+  SMX_REQUIRE(p && R_grid && z_grid && phi_grid && sample && out && P >= 0 && S > 0,
+              "smx_density_target: bad arguments");
+  if (P == 0) return 0;
+  hipLaunchKernelGGL(density_target_kernel, dim3((unsigned)P), dim3(256), 0, as_stream(stream), *p,
+                     R_grid, z_grid, phi_grid, P, dR, dz, dphi, sample, S, out);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"

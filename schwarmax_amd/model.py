@@ -1,0 +1,262 @@
+"""Forward model on the GPU: build_model / get_model_with_orbit drop-ins.
+
+Same signatures, argument meaning and output keys as model.py:70-286
+(build_model) and model.py:606-850 (get_model_with_orbit).  Every per-orbit,
+per-cell and per-iteration computation runs in the HIP kernels of libsmx.so;
+torch holds device memory and does the few O(K*B) elementwise steps between
+kernels (data normalisation, clipping, V/sigma).
+
+Two inputs the reference draws internally from fixed JAX PRNG keys
+(model.py:112, 128-138) are explicit here, because JAX's threefry stream
+cannot be reproduced without JAX: ``dict_data['jeans_normals']`` (n, 3) and
+``dict_data['jitter_uniforms']`` (n, r, 6).  The integration schedule that
+build_model hard-codes (model.py:163-169) is the default of ``schedule``.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import potentials as POT
+from .utils import makeRotationMatrix
+from .ghmoments import h_to_V_sigma
+
+EPSILON = 1e-12
+DEFAULT_SCHEDULE = dict(N_step_per_orb=100, N_dynamical_time=50, N_max=None, atol=1e-7,
+                        rtol=1e-4, dt_min=1e-5, dt_max=0.3, chunk=100, n_realizations=4)
+
+
+def schedule_of(sched):
+    s = dict(DEFAULT_SCHEDULE)
+    if sched:
+        s.update(sched)
+    if s['N_max'] is None:
+        s['N_max'] = int(s['N_step_per_orb'] * s['N_dynamical_time'])
+    return s
+
+
+def _canonical(density_func, potential_func):
+    from . import likelihood
+    if density_func is not likelihood.density_func or potential_func is not likelihood.potential_func:
+        raise NotImplementedError(
+            "the GPU path implements the canonical density_func / potential_func of "
+            "likelihood.py (NFW + Miyamoto-Nagai + T3 bar + V4 bulge); pass "
+            "schwarmax_amd.likelihood.density_func / potential_func")
+
+
+def baryon_params(params_disk_rho):
+    """params_baryon of model.py:87-102."""
+    keys = ('logM_disc', 'Rs_disc', 'Hs_disc', 'logM_bar', 'L_bar', 'a_bar', 'b_bar',
+            'x_origin', 'y_origin', 'z_origin', 'dirx', 'diry', 'dirz')
+    return {k: params_disk_rho[k] for k in keys}
+
+
+def make_grids(num_Vbin, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, rot, symmetry):
+    g = _lib.SmxGrids()
+    nR, nz, nphi = (int(v) for v in np.asarray(Rzphi_n_grid).ravel())
+    nX, nY = (int(v) for v in np.asarray(xy_n_grid).ravel())
+    lim = np.asarray(Rzphi_lim_grid, dtype=np.float64)
+    xyl = np.asarray(xy_lim_grid, dtype=np.float64)
+    g.nR, g.nz, g.nphi, g.nX, g.nY = nR, nz, nphi, nX, nY
+    g.num_vbin = int(num_Vbin)
+    g.symmetry = 1 if symmetry else 0
+    g.rz_min[:] = list(lim[:, 0])
+    g.rz_max[:] = list(lim[:, 1])
+    g.xy_min[:] = list(xyl[:, 0])
+    g.xy_max[:] = list(xyl[:, 1])
+    g.rot[:] = list(np.asarray(rot, dtype=np.float64).ravel())
+    # integrates.py:70-71
+    g.area_pixel = ((xyl[0, 1] - xyl[0, 0]) / nX) * ((xyl[1, 1] - xyl[1, 0]) / nY) * 1e6
+    return g
+
+
+def make_schedule(s, Omega):
+    sc = _lib.SmxSchedule()
+    sc.N_max = int(s['N_max'])
+    sc.Omega = float(Omega)
+    sc.atol, sc.rtol = float(s['atol']), float(s['rtol'])
+    sc.dt_min, sc.dt_max = float(s['dt_min']), float(s['dt_max'])
+    return sc
+
+
+def initial_conditions(pot, dict_data, r, sched):
+    """Jeans ICs, jitter and time scales on the GPU (model.py:112-169)."""
+    L = _lib.lib()
+    w0 = _lib.dev(np.asarray(dict_data['w0'])[:, :3])
+    n = w0.shape[0]
+    normals = _lib.dev(dict_data['jeans_normals'])
+    uni = _lib.dev(dict_data['jitter_uniforms'])
+    r_total = uni.shape[1]
+    wb = torch.empty((n, r, 6), dtype=torch.float64, device='cuda')
+    Tt = torch.empty((n, r), dtype=torch.float64, device='cuda')
+    dt0 = torch.empty((n, r), dtype=torch.float64, device='cuda')
+    _lib.check(L.smx_initial_conditions(pot, _lib.ptr(w0), _lib.ptr(normals), _lib.ptr(uni), n, r,
+                                        r_total, float(sched['N_step_per_orb']),
+                                        float(sched['N_dynamical_time']), _lib.ptr(wb), _lib.ptr(Tt),
+                                        _lib.ptr(dt0), _lib.stream_ptr()), 'smx_initial_conditions')
+    return wb, Tt, dt0
+
+
+def build_library(pot, grids, sched_struct, wb, Tt, dt0, dict_data, traj=False):
+    """Orbit library (n, P + 6B) columns [Rzphi | xy | h1 | h2 | h3 | h4 | h0]."""
+    L = _lib.lib()
+    n, r = wb.shape[0], wb.shape[1]
+    n_int = n * r
+    P = grids.nR * grids.nz * grids.nphi
+    B = grids.num_vbin
+    ws = torch.empty(int(L.smx_library_workspace_bytes(grids, n_int)), dtype=torch.uint8, device='cuda')
+    lib = torch.empty((n, P + 6 * B), dtype=torch.float64, device='cuda')
+    valid = torch.empty(n_int, dtype=torch.float64, device='cuda')
+    T_int = torch.empty(n_int, dtype=torch.float64, device='cuda')
+    bm = _lib.dev(dict_data['bin_mapping'], dtype=torch.int32)
+    npb = _lib.dev(dict_data['num_per_bin'])
+    v0 = _lib.dev(dict_data['v0'])
+    sd = _lib.dev(dict_data['s'])
+    if bm.numel() < grids.nX * grids.nY:
+        raise ValueError("bin_mapping must have nX*nY entries (plus the sentinel)")
+    extra = {}
+    if traj:
+        N = sched_struct.N_max
+        o_traj = torch.empty((n_int, N, 6), dtype=torch.float64, device='cuda')
+        t_traj = torch.empty((n_int, N), dtype=torch.float64, device='cuda')
+        _lib.check(L.smx_build_library_traj(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),
+                                            _lib.ptr(dt0), n, r, _lib.ptr(bm), _lib.ptr(npb),
+                                            _lib.ptr(v0), _lib.ptr(sd), _lib.ptr(ws), _lib.ptr(lib),
+                                            _lib.ptr(o_traj), _lib.ptr(t_traj), _lib.ptr(valid),
+                                            _lib.ptr(T_int), _lib.stream_ptr()),
+                   'smx_build_library_traj')
+        extra = {'o_traj': o_traj, 't_traj': t_traj}
+    else:
+        t_final = torch.empty(n_int, dtype=torch.float64, device='cuda')
+        _lib.check(L.smx_build_library(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),
+                                       _lib.ptr(dt0), n, r, _lib.ptr(bm), _lib.ptr(npb), _lib.ptr(v0),
+                                       _lib.ptr(sd), _lib.ptr(ws), _lib.ptr(lib), _lib.ptr(valid),
+                                       _lib.ptr(T_int), _lib.ptr(t_final), _lib.stream_ptr()),
+                   'smx_build_library')
+        extra = {'t_final': t_final}
+    del ws
+    return dict(lib=lib, valid=valid, T=T_int, P=P, B=B, **extra)
+
+
+def density_target(params_disk_rho, dict_data):
+    """y_Rzphi by Sobol quadrature on the GPU (model.py:192-213)."""
+    L = _lib.lib()
+    Rg = _lib.dev(dict_data['R_grid'])
+    zg = _lib.dev(dict_data['z_grid'])
+    pg = _lib.dev(dict_data['phi_grid'])
+    smp = _lib.dev(dict_data['sample_for_integration'])
+    P = Rg.numel()
+    out = torch.empty(P, dtype=torch.float64, device='cuda')
+    _lib.check(L.smx_density_target(POT.pack_potential(params_disk_rho), _lib.ptr(Rg), _lib.ptr(zg),
+                                    _lib.ptr(pg), P, float(dict_data['dR']), float(dict_data['dz']),
+                                    float(dict_data['dphi']), _lib.ptr(smp), smp.shape[0],
+                                    _lib.ptr(out), _lib.stream_ptr()), 'smx_density_target')
+    return out
+
+
+def compute_model_single(W, lib, P, B, gamma_kin, v0, s):
+    """Model maps for weight rows W (K, n) (model.py:14-66)."""
+    from .nnls import model_maps
+    m = P + 5 * B
+    maps = model_maps(lib, m, W)
+    g = torch.where(gamma_kin.abs() > 1e-8, gamma_kin, torch.full_like(gamma_kin, 1e-8))
+    d3 = maps[:, :P]
+    d2 = maps[:, P:P + B]
+    h = [torch.clamp(maps[:, P + B * (k + 1):P + B * (k + 2)] / g, -10.0, 10.0) for k in range(4)]
+    V, sig = h_to_V_sigma(h[0], h[1], v0, s, h3=h[2], h4=h[3])
+    return d3, d2, h[0], h[1], h[2], h[3], V, sig
+
+
+def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
+             Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, nnls_maxiter,
+             regularization, schedule, traj, channel_w, nnls_tol):
+    from .nnls import solve_nnls_admm_bootstrap_lib
+    _canonical(density_func, potential_func)
+    sched = schedule_of(schedule)
+    pb = baryon_params(params_disk_rho)
+    pot = POT.pack_potential(pb, params_halo_pot)
+    lm = 1.0 / float(params_disk_rho['mass_to_light_ratio'])
+    Omega_bar = float(params_disk_rho['Omega_bar'])
+    rot = makeRotationMatrix(float(params_disk_rho['alpha']), float(params_disk_rho['beta']),
+                             float(params_disk_rho['gamma']))
+    r = 1 if traj else int(sched['n_realizations'])
+    grids = make_grids(num_Vbin, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, rot,
+                       symmetry=not traj)
+    if grids.nR * grids.nz * grids.nphi != int(Rzphi_n_tot):
+        raise ValueError("Rzphi_n_tot must equal prod(Rzphi_n_grid)")
+    wb, Tt, dt0 = initial_conditions(pot, dict_data, r, sched)
+    libd = build_library(pot, grids, make_schedule(sched, -Omega_bar), wb, Tt, dt0, dict_data, traj)
+    lib, P, B = libd['lib'], libd['P'], libd['B']
+    n = lib.shape[0]
+
+    y_R = density_target(params_disk_rho, dict_data)
+    sig_R = 0.02 * y_R + 1e-10
+    D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
+    y_xy = D('XY_density_data') / lm
+    sig_xy = (D('XY_density_data_err') + EPSILON) / lm
+    y_h = [D('h%d_data' % k) for k in (1, 2, 3, 4)]
+    sig_h = [D('h%d_data_err' % k) + EPSILON for k in (1, 2, 3, 4)]
+    mmpo = torch.sum(y_R) / n                                   # model.py:231-235
+    y_xy = y_xy / mmpo
+    sig_xy = sig_xy / mmpo
+    y_R = y_R / mmpo
+    sig_R = sig_R / mmpo
+    Xm = np.asarray(dict_data['X_minmax'], dtype=np.float64)
+    Ym = np.asarray(dict_data['Y_minmax'], dtype=np.float64)
+    nXY = np.asarray(dict_data['nX_nY'])
+    area = ((Xm[1] - Xm[0]) / nXY[0]) * ((Ym[1] - Ym[0]) / nXY[1]) * 1e6
+    npb = D('num_per_bin')
+    s_ap, v0_ap = D('s'), D('v0')
+    M_per_bin = y_xy * npb * area
+    norm_GH = 1.0 + y_h[3] * np.sqrt(6.0) / 4.0
+    gamma_kin = M_per_bin / (norm_GH * 2.0 * np.sqrt(np.pi) * s_ap + EPSILON)
+    y_xy_boot = y_xy[None, :] + D('XY_standard_normal') * sig_xy[None, :]
+    y_h_boot = [y_h[k][None, :] + D('h%d_standard_normal' % (k + 1)) * sig_h[k][None, :]
+                for k in range(4)]
+    W, info = solve_nnls_admm_bootstrap_lib(lib, P, B, y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R,
+                                            sig_xy, sig_h, lambda_reg=regularization,
+                                            maxiter=nnls_maxiter, w_rzphi=channel_w[0],
+                                            w_xy=channel_w[1], w_h=channel_w[2], tol=nnls_tol)
+    d3, d2, h1, h2, h3, h4, V, sig = compute_model_single(W, lib, P, B, gamma_kin, v0_ap, s_ap)
+    d2 = d2 * mmpo * lm
+    out = {'density_3d': y_R, 'density_3d_model': d3, 'surface_density': d2, 'h1': h1, 'h2': h2,
+           'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
+    aux = dict(libd=libd, mmpo=mmpo, rot=rot, Omega_bar=Omega_bar, n=n, nnls_iters=info['iters'],
+               wb=wb, T_total=Tt, dt0=dt0)
+    return out, aux
+
+
+def build_model(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
+                Rzphi_n_tot=360, Rzphi_n_grid=(10, 6, 6),
+                Rzphi_lim_grid=((0, 10.), (-3, 3), (-np.pi, np.pi)),
+                xy_lim_grid=((-10., 10.), (-3., 3.)), xy_n_grid=(60, 40),
+                nnls_maxiter=200, regularization=1.0, schedule=None, nnls_tol=0.0,
+                return_aux=False):
+    """model.py:70-286 on the GPU.  Returns the reference's dict (torch, on device)."""
+    out, aux = _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data,
+                        num_Vbin, Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid,
+                        nnls_maxiter, regularization, schedule, False,
+                        (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0)), nnls_tol)
+    return (out, aux) if return_aux else out
+
+
+def get_model_with_orbit(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data,
+                         num_Vbin, Rzphi_n_tot=360, Rzphi_n_grid=(10, 6, 6),
+                         Rzphi_lim_grid=((0, 10.), (-3, 3), (-np.pi, np.pi)),
+                         xy_lim_grid=((-10., 10.), (-3., 3.)), xy_n_grid=(60, 40),
+                         nnls_maxiter=200, regularization=1.0, schedule=None, nnls_tol=0.0):
+    """model.py:606-850 on the GPU (one realization, trajectories kept)."""
+    out, aux = _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data,
+                        num_Vbin, Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid,
+                        nnls_maxiter, regularization, schedule, True, (1.0, 1.0, 1.0), nnls_tol)
+    libd = aux['libd']
+    n = aux['n']
+    N = libd['o_traj'].shape[1]
+    res = {
+        'weights': out['weights'][0], 'weights_all_boot': out['weights'],
+        'y_traj': libd['o_traj'].reshape(n, 1, N, 6), 't_traj': libd['t_traj'].reshape(n, 1, N),
+        'rotation_matrix': aux['rot'], 'Omega_bar': aux['Omega_bar'], 'n_particles': n,
+        'n_realizations': 1, 'mass_per_orbit': aux['mmpo'],
+    }
+    for k in ('density_3d', 'density_3d_model', 'surface_density', 'h1', 'h2', 'h3', 'h4', 'V', 'sigma'):
+        res[k] = out[k]
+    return res

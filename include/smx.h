@@ -67,6 +67,14 @@ typedef struct smx_schedule {
 /* ---- library management ---------------------------------------------- */
 int smx_last_error(char* buf, int64_t len);
 int smx_version(void);
+/* Per-stage timing with HIP events on the launch stream (no counterpart in
+ * the reference; used by the benchmark).  Stages, in order: initial
+ * conditions, integration, finalize, Gram, factor + inverse, right-hand sides,
+ * ADMM loop, model maps, density target.  enable != 0 turns recording on and
+ * clears previous records; read fills ms[] and launches[] (kernel launches per
+ * stage) summed over the records since, then clears them. */
+int smx_timing_enable(int enable);
+int smx_timing_read(double* ms, int64_t* launches, int32_t max_stages);
 
 /* ---- potential (potentials.py, dehnen_bar.py, likelihood.py:13-71) ------ */
 /* Phi and acc = -grad Phi at n points xyz (n,3); acc may be NULL.
@@ -103,9 +111,8 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
                       void* workspace, double* lib, double* valid, double* T_int,
                       double* t_final, void* stream);
 /* Trajectory variant, no symmetry, dt normalised (integrates.py:319-581):
- * o_traj (n_int, N_max, 6) with velocities in km/s, t_traj (n_int, N_max),
- * per-integration library rows lib_int (n_int, P + 6B) and the realization
- * average lib (n, P + 6B). */
+ * o_traj (n_int, N_max, 6) with velocities in km/s, t_traj (n_int, N_max)
+ * the cumulative time, and the realization average lib (n, P + 6B). */
 int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                            const double* wb, const double* T_total, const double* dt0,
                            int64_t n, int32_t r, const int32_t* bin_mapping,
@@ -145,6 +152,21 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
                   const double* row_scale, const double* Y, int64_t K,
                   double lambda_reg, int32_t maxiter, double w_init, double tol,
                   void* workspace, double* W, int32_t* iters_done, void* stream);
+
+/* ---- dense building blocks (no reference counterpart: the reference calls
+ * XLA's dot / cholesky / cho_solve; exposed for tests) ------------------- */
+/* C = alpha op(A) op(B) + beta C in FP32 on MFMA; op(A) is (M,K), op(B) (K,N);
+ * trans_a: A stored (K, M); trans_b: B stored (N, K); not both. */
+int smx_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, float alpha,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+                 int64_t ldc, void* stream);
+/* Same in FP64 on MFMA. */
+int smx_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                 int64_t ldc, void* stream);
+/* In-place inverse of a full symmetric positive definite n x n FP64 matrix
+ * (recursive Schur complements); work holds n*n/2 + n doubles. */
+int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream);
 
 /* ---- model maps and likelihood (model.py:14-68, likelihood.py:199-244) -- */
 /* maps (K, m) = W (K, n) . lib[:, :m]^T in FP64. */

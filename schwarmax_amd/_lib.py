@@ -58,6 +58,8 @@ _D = ctypes.c_double
 _SIGS = [
     ('smx_last_error', ctypes.c_int, [ctypes.c_char_p, _I64]),
     ('smx_version', ctypes.c_int, []),
+    ('smx_timing_enable', ctypes.c_int, [ctypes.c_int]),
+    ('smx_timing_read', ctypes.c_int, [_P, _P, _I32]),
     ('smx_potential_eval', ctypes.c_int, [ctypes.POINTER(SmxPotential), _P, _I64, _P, _P, _P]),
     ('smx_density_eval', ctypes.c_int, [ctypes.POINTER(SmxPotential), _P, _I64, _P, _P]),
     ('smx_initial_conditions', ctypes.c_int,
@@ -76,6 +78,11 @@ _SIGS = [
     ('smx_nnls_workspace_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_admm', ctypes.c_int,
      [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
+    ('smx_gemm_f32', ctypes.c_int, [_I32, _I32, _I64, _I64, _I64, ctypes.c_float, _P, _I64, _P,
+                                    _I64, ctypes.c_float, _P, _I64, _P]),
+    ('smx_gemm_f64', ctypes.c_int, [_I32, _I32, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P,
+                                    _I64, _P]),
+    ('smx_spd_inverse_f64', ctypes.c_int, [_P, _I64, _P, _P]),
     ('smx_model_maps', ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _P, _P]),
 ]
 
@@ -137,3 +144,18 @@ def dev(a, dtype=torch.float64, device='cuda'):
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=dtype).contiguous()
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+
+
+STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps', 'density')
+
+
+def timing_enable(on=True):
+    check(load_library().smx_timing_enable(1 if on else 0), 'smx_timing_enable')
+
+
+def timing_read():
+    """{stage: (ms, launches)} summed since the last enable/read."""
+    ms = (ctypes.c_double * len(STAGES))()
+    ln = (ctypes.c_int64 * len(STAGES))()
+    check(load_library().smx_timing_read(ms, ln, len(STAGES)), 'smx_timing_read')
+    return {s: (ms[i], ln[i]) for i, s in enumerate(STAGES)}

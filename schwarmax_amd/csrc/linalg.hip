@@ -3,15 +3,18 @@
 //
 // Device pipeline for K right-hand sides:
 //   U^T (n, m)        FP32, from the FP64 library columns and per-row scales
-//   Q = U^T U         FP32 GEMM (n, n); + lambda/n I; rho = tr(Q)/n
-//   L L^T = Q + rho I blocked right-looking Cholesky (FP32)
-//   Z = L^-1          blocked triangular inverse
-//   M = Z^T Z         = (Q + rho I)^-1, so one ADMM iteration is ONE GEMM
+//   Q = U^T U         FP32 MFMA GEMM (n, n), widened to FP64; + lambda/n I;
+//                     rho = tr(Q)/n
+//   M = (Q + rho I)^-1  FP64, in place, recursive Schur complements on FP64
+//                     MFMA GEMMs
+//                     (the reference factors Q + rho I once and solves with
+//                     the factor every iteration; the explicit inverse makes
+//                     each iteration one product)
 //   C = -Y U          right-hand-side terms (K, n)
-//   per iteration:    W = R M  (R = rho (z - u) - c), epilogue does the
-//                     relaxation / projection / dual update of nnls.py:152-157
-//                     and writes the next R.
-// z and u are kept in FP64; the products run in FP32 with FP32 accumulation.
+//   per iteration:    W = R M  (R = rho (z - u) - c) on fp16-split MFMA, then
+//                     the relaxation / projection / dual update of
+//                     nnls.py:152-157, which also writes the next R.
+// z and u are kept in FP64; products carry FP32-level rounding.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -20,96 +23,189 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "timing.hpp"
 #include "../../include/smx.h"
 
 namespace smx {
 
 // ---------------------------------------------------------------------------
-// generic tiled SGEMM:  C = alpha * op(A) op(B) + beta * C
+// GEMM conventions:  C = alpha * op(A) op(B) + beta * C
 //   op(A) is (M, K): A[i*lda + k] (TA = false) or A[k*lda + i] (TA = true)
 //   op(B) is (K, N): B[k*ldb + j] (TB = false) or B[j*ldb + k] (TB = true)
-// kmode: 0 full k range; 1 lower-triangular A (k <= i); 2 k >= max(i, j)
-//        (product of two lower-triangular operands Z^T Z).
 // lower_only: skip tiles strictly above the diagonal.
 // ---------------------------------------------------------------------------
-constexpr int BM = 64, BN = 64, BK = 16;
+// ---------------------------------------------------------------------------
+// FP32 GEMM on MFMA (v_mfma_f32_32x32x2f32), 128 x 128 tiles, 4 waves each
+// owning 64 x 64 of the output.
+// ---------------------------------------------------------------------------
+typedef float f32x16m __attribute__((ext_vector_type(16)));
 
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, float alpha,
-                                                    const float* __restrict__ A, int64_t lda,
-                                                    const float* __restrict__ B, int64_t ldb,
-                                                    float beta, float* __restrict__ C, int64_t ldc,
-                                                    int kmode, int lower_only) {
-  const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
-  if (lower_only && j0 > i0 + BM - 1) return;
-  __shared__ float As[BK][BM + 4];
-  __shared__ float Bs[BK][BN + 4];
-  const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  float acc[4][4] = {};
-  int kbeg = 0, kend = K;
-  if (kmode == 1) kend = min(K, i0 + BM);
-  if (kmode == 2) kbeg = (max(i0, j0) / BK) * BK;
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    // A tile: BM x BK
+__global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(int M, int N, int K, float alpha,
+                                                           const float* __restrict__ A, int64_t lda,
+                                                           const float* __restrict__ B, int64_t ldb,
+                                                           float beta, float* __restrict__ C,
+                                                           int64_t ldc, int lower_only) {
+  constexpr int TM = 128, TN = 128, TK = 32;
+  __shared__ float As[TK][TM + 4];
+  __shared__ float Bs[TK][TN + 4];
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  if (lower_only && j0 > i0 + TM - 1) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 5, c = lane & 31;
+  const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+  f32x16m acc[2][2];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
       const int e = tid + 256 * l;
       int ii, kk;
-      if (TA) { ii = e & (BM - 1); kk = e >> 6; } else { kk = e & (BK - 1); ii = e >> 4; }
+      if (TA) { ii = e & (TM - 1); kk = e >> 7; } else { kk = e & (TK - 1); ii = e >> 5; }
       const int gi = i0 + ii, gk = k0 + kk;
       float v = 0.f;
-      if (gi < M && gk < kend) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
+      if (gi < M && gk < K) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
       As[kk][ii] = v;
     }
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
+    for (int l = 0; l < 16; ++l) {
       const int e = tid + 256 * l;
       int jj, kk;
-      if (TB) { kk = e & (BK - 1); jj = e >> 4; } else { jj = e & (BN - 1); kk = e >> 6; }
+      if (TB) { kk = e & (TK - 1); jj = e >> 5; } else { jj = e & (TN - 1); kk = e >> 7; }
       const int gj = j0 + jj, gk = k0 + kk;
       float v = 0.f;
-      if (gj < N && gk < kend) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
+      if (gj < N && gk < K) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
       Bs[kk][jj] = v;
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < BK; ++kk) {
-      float a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][ty * 4 + r];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+    for (int kk = 0; kk < TK; kk += 2) {
+      const float a0 = As[kk + g][wi + c], a1 = As[kk + g][wi + 32 + c];
+      const float b0 = Bs[kk + g][wj + c], b1 = Bs[kk + g][wj + 32 + c];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int gi = i0 + ty * 4 + r;
-    if (gi >= M) continue;
+  for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int gj = j0 + tx * 4 + c;
-      if (gj >= N) continue;
-      if (lower_only && gj > gi) continue;
-      float* p = C + (int64_t)gi * ldc + gj;
-      *p = (beta == 0.f) ? alpha * acc[r][c] : alpha * acc[r][c] + beta * (*p);
-    }
-  }
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gi = i0 + wi + rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        const int gj = j0 + wj + cb * 32 + c;
+        if (gi >= M || gj >= N) continue;
+        if (lower_only && gj > gi) continue;
+        float* p = C + (int64_t)gi * ldc + gj;
+        *p = (beta == 0.f) ? alpha * acc[rb][cb][r] : alpha * acc[rb][cb][r] + beta * (*p);
+      }
 }
 
 template <bool TA, bool TB>
-static int sgemm(hipStream_t st, int M, int N, int K, float alpha, const float* A, int64_t lda,
-                 const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int kmode = 0,
-                 int lower_only = 0) {
+static int gemm(hipStream_t st, int M, int N, int K, float alpha, const float* A, int64_t lda,
+                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int lower_only = 0) {
   if (M <= 0 || N <= 0) return 0;
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-  hipLaunchKernelGGL((sgemm_kernel<TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B,
-                     ldb, beta, C, ldc, kmode, lower_only);
+  dim3 grid((N + 127) / 128, (M + 127) / 128);
+  hipLaunchKernelGGL((gemm_mfma_kernel<TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B,
+                     ldb, beta, C, ldc, lower_only);
+  SMX_LAUNCHED();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64), 128 x 128 tiles, 4 waves each
+// owning 64 x 64 (4 x 4 blocks of 16 x 16); same operand conventions.
+// ---------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void dgemm_mfma_kernel(int M, int N, int K, double alpha,
+                                                         const double* __restrict__ A, int64_t lda,
+                                                         const double* __restrict__ B, int64_t ldb,
+                                                         double beta, double* __restrict__ C,
+                                                         int64_t ldc) {
+  constexpr int TM = 128, TN = 128, TK = 16;
+  __shared__ double As[TK][TM + 2];
+  __shared__ double Bs[TK][TN + 2];
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int e = tid + 256 * l;
+      int ii, kk;
+      if (TA) { ii = e & (TM - 1); kk = e >> 7; } else { kk = e & (TK - 1); ii = e >> 4; }
+      const int gi = i0 + ii, gk = k0 + kk;
+      double v = 0.0;
+      if (gi < M && gk < K) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
+      As[kk][ii] = v;
+    }
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int e = tid + 256 * l;
+      int jj, kk;
+      if (TB) { kk = e & (TK - 1); jj = e >> 4; } else { jj = e & (TN - 1); kk = e >> 7; }
+      const int gj = j0 + jj, gk = k0 + kk;
+      double v = 0.0;
+      if (gj < N && gk < K) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
+      Bs[kk][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = As[kk + g][wi + r * 16 + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) b[r] = Bs[kk + g][wj + r * 16 + c];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // v_mfma_f64_16x16x4f64 accumulator layout: lane (g, c), element r ->
+  // row g + 4r, column c
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wi + rb * 16 + g + 4 * r;
+        const int gj = j0 + wj + cb * 16 + c;
+        if (gi >= M || gj >= N) continue;
+        double* p = C + (int64_t)gi * ldc + gj;
+        *p = (beta == 0.0) ? alpha * acc[rb][cb][r] : alpha * acc[rb][cb][r] + beta * (*p);
+      }
+}
+
+template <bool TA, bool TB>
+static int dgemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return 0;
+  dim3 grid((N + 127) / 128, (M + 127) / 128);
+  hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B,
+                     ldb, beta, C, ldc);
   SMX_LAUNCHED();
   return 0;
 }
@@ -135,211 +231,9 @@ __global__ void to_float_kernel(const double* __restrict__ src, int64_t rows, in
   dst[idx] = (j < cols) ? static_cast<float>(src[i * cols + j]) : 0.f;
 }
 
-// Q += reg I; rho = tr(Q)/n; Q += rho I.  One block.
-__global__ __launch_bounds__(1024) void regularise_kernel(float* __restrict__ Q, int64_t n,
-                                                          float reg, float* __restrict__ rho_out) {
-  __shared__ double red[16];
-  double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    const float d = Q[i * n + i] + reg;
-    Q[i * n + i] = d;
-    acc += d;
-  }
-  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  __shared__ float rho_s;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    rho_s = static_cast<float>(t) / static_cast<float>(n);
-    *rho_out = rho_s;
-  }
-  __syncthreads();
-  const float rho = rho_s;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Q[i * n + i] = Q[i * n + i] + rho;
-}
-
 // ---------------------------------------------------------------------------
-// blocked Cholesky (lower, in place, row-major), block NB
+// ADMM helpers
 // ---------------------------------------------------------------------------
-constexpr int NB = 64;
-
-// factor the jb x jb diagonal block at (k, k); one block of NB*4 threads
-__global__ __launch_bounds__(256) void potrf_diag_kernel(float* __restrict__ A, int64_t lda, int64_t k,
-                                                         int jb) {
-  __shared__ float T[NB][NB + 1];
-  for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-    const int r = e / jb, c = e - r * jb;
-    T[r][c] = A[(k + r) * lda + k + c];
-  }
-  __syncthreads();
-  for (int j = 0; j < jb; ++j) {
-    if (threadIdx.x == 0) T[j][j] = sqrtf(T[j][j]);
-    __syncthreads();
-    const float d = T[j][j];
-    for (int r = j + 1 + threadIdx.x; r < jb; r += blockDim.x) T[r][j] = T[r][j] / d;
-    __syncthreads();
-    // trailing update of the lower triangle
-    for (int e = threadIdx.x; e < (jb - j - 1) * (jb - j - 1); e += blockDim.x) {
-      const int r = j + 1 + e / (jb - j - 1), c = j + 1 + e % (jb - j - 1);
-      if (c <= r) T[r][c] -= T[r][j] * T[c][j];
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-    const int r = e / jb, c = e - r * jb;
-    A[(k + r) * lda + k + c] = (c <= r) ? T[r][c] : 0.f;
-  }
-}
-
-// rows i in [k+jb, n): L21[i,:] = A21[i,:] L11^-T (forward substitution)
-__global__ __launch_bounds__(256) void trsm_panel_kernel(float* __restrict__ A, int64_t lda, int64_t n,
-                                                         int64_t k, int jb) {
-  __shared__ float L[NB][NB + 1];
-  for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-    const int r = e / jb, c = e - r * jb;
-    L[r][c] = A[(k + r) * lda + k + c];
-  }
-  __syncthreads();
-  const int64_t i = k + jb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float x[NB];
-  float* row = A + i * lda + k;
-  for (int j = 0; j < jb; ++j) {
-    float s = row[j];
-    for (int l = 0; l < j; ++l) s -= x[l] * L[j][l];
-    x[j] = s / L[j][j];
-  }
-  for (int j = 0; j < jb; ++j) row[j] = x[j];
-}
-
-// zero the strict upper triangle of an n x n row-major matrix
-__global__ void zero_upper_kernel(float* __restrict__ A, int64_t n) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int64_t i = idx / n, j = idx - i * n;
-  if (j > i) A[idx] = 0.f;
-}
-
-// invert the lower-triangular jb x jb diagonal block at (j, j) in place
-__global__ __launch_bounds__(64) void trti_diag_kernel(float* __restrict__ A, int64_t lda, int64_t j0,
-                                                       int jb) {
-  __shared__ float L[NB][NB + 1];
-  __shared__ float X[NB][NB + 1];
-  for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-    const int r = e / jb, c = e - r * jb;
-    L[r][c] = A[(j0 + r) * lda + j0 + c];
-  }
-  __syncthreads();
-  // column c of X solves L x = e_c; thread c owns column c
-  const int c = threadIdx.x;
-  if (c < jb) {
-    for (int r = 0; r < jb; ++r) {
-      float s = (r == c) ? 1.f : 0.f;
-      for (int l = c; l < r; ++l) s -= L[r][l] * X[l][c];
-      X[r][c] = (r < c) ? 0.f : s / L[r][r];
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-    const int r = e / jb, cc = e - r * jb;
-    A[(j0 + r) * lda + j0 + cc] = X[r][cc];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// ADMM
-// ---------------------------------------------------------------------------
-// state init: z = w_init, u = 0, R = rho (z - u) - c
-__global__ void admm_init_kernel(int64_t total, double w_init, const float* __restrict__ rho_p,
-                                 const float* __restrict__ c, double* __restrict__ z,
-                                 double* __restrict__ u, float* __restrict__ R) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const double rho = *rho_p;
-  z[i] = w_init;
-  u[i] = 0.0;
-  R[i] = static_cast<float>(rho * (w_init - 0.0) - static_cast<double>(c[i]));
-}
-
-// Wt = R M (R: (K, n), M symmetric (n, n)) with the ADMM update fused in the
-// epilogue (nnls.py:152-157).  delta[k] receives max |z_new - z| bits.
-__global__ __launch_bounds__(256) void admm_step_kernel(int K, int n, const float* __restrict__ R,
-                                                        const float* __restrict__ M,
-                                                        const float* __restrict__ rho_p,
-                                                        const float* __restrict__ c,
-                                                        double* __restrict__ z,
-                                                        double* __restrict__ u,
-                                                        float* __restrict__ Rn, double alpha,
-                                                        const int* __restrict__ active,
-                                                        unsigned* __restrict__ delta) {
-  const int i0 = blockIdx.y * BM, j0 = blockIdx.x * BN;
-  __shared__ float As[BK][BM + 4];
-  __shared__ float Bs[BK][BN + 4];
-  const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  float acc[4][4] = {};
-  for (int k0 = 0; k0 < n; k0 += BK) {
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int e = tid + 256 * l;
-      const int kk = e & (BK - 1), ii = e >> 4;
-      const int gi = i0 + ii, gk = k0 + kk;
-      As[kk][ii] = (gi < K && gk < n) ? R[(int64_t)gi * n + gk] : 0.f;
-    }
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int e = tid + 256 * l;
-      const int kk = e & (BK - 1), jj = e >> 4;
-      const int gj = j0 + jj, gk = k0 + kk;
-      Bs[kk][jj] = (gj < n && gk < n) ? M[(int64_t)gj * n + gk] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; ++kk) {
-      float a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][ty * 4 + r];
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) b[cc] = Bs[kk][tx * 4 + cc];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) acc[r][cc] += a[r] * b[cc];
-    }
-    __syncthreads();
-  }
-  const double rho = *rho_p;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int gi = i0 + ty * 4 + r;
-    if (gi >= K) continue;
-    const bool act = active[gi] != 0;
-    float dmax = 0.f;
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-      const int gj = j0 + tx * 4 + cc;
-      if (gj >= n) continue;
-      const int64_t o = (int64_t)gi * n + gj;
-      if (act) {
-        const double w = acc[r][cc];
-        const double zo = z[o], uo = u[o];
-        const double wh = alpha * w + (1.0 - alpha) * zo;
-        const double zn = fmax(0.0, wh + uo);
-        const double un = (uo + wh) - zn;
-        z[o] = zn;
-        u[o] = un;
-        Rn[o] = static_cast<float>(rho * (zn - un) - static_cast<double>(c[o]));
-        dmax = fmaxf(dmax, static_cast<float>(fabs(zn - zo)));
-      } else {
-        Rn[o] = R[o];
-      }
-    }
-    if (act && dmax > 0.f) atomicMax(delta + gi, __float_as_uint(dmax));
-  }
-}
-
 __global__ void zmax_kernel(int K, int64_t n, const double* __restrict__ z, unsigned* __restrict__ zmax) {
   const int k = blockIdx.y;
   float m = 0.f;
@@ -351,6 +245,209 @@ __global__ void zmax_kernel(int K, int64_t n, const double* __restrict__ z, unsi
 __global__ void copy_out_kernel(int64_t total, const double* __restrict__ z, double* __restrict__ W) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < total) W[i] = z[i];
+}
+
+// ---------------------------------------------------------------------------
+// ADMM product on MFMA.  Each iteration needs W^T = R M with R (K, n) and
+// M = (Q + rho I)^-1 (n, n), symmetric.  Operands are split into fp16 pairs
+// x = s (hi + lo) with power-of-two scales (M: one scale from rho, since
+// |M_ij| <= 1/rho; R: one scale per row from its max), and the product is
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32 accumulation.
+// Every fp16 product is exact in FP32, so the result carries FP32-level
+// error; the dropped lo*lo term is 2^-22 relative.
+//
+// Work split: a workgroup (4 waves) owns 128 rows of R x 256 columns of M and
+// a contiguous range of the reduction index i; it writes its partial sum (no
+// atomics).  The update kernel adds the partials and applies the ADMM step.
+// Symmetry of M lets the B operand be read along rows: B[i][j] = M[j][i].
+// ---------------------------------------------------------------------------
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int AG_KR = 128;   // rows of R per workgroup
+constexpr int AG_BN = 256;   // columns of M per workgroup
+constexpr int AG_KC = 64;    // reduction chunk
+
+__device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
+  // 2^(target - ceil(log2 amax)), or 1 for an all-zero row
+  if (!(amax > 0.f)) return 1.f;
+  int e;
+  frexpf(amax, &e);            // amax = f * 2^e, f in [0.5, 1)
+  return ldexpf(1.f, target_exp - e);
+}
+
+// M (n, n) FP32 -> scaled fp16 hi / lo (npad, npad), zero padded.
+__global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t npad,
+                               const float* __restrict__ rho_p, _Float16* __restrict__ Mh,
+                               _Float16* __restrict__ Ml) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= npad * npad) return;
+  const int64_t i = idx / npad, j = idx - i * npad;
+  const double sc = pow2_scale_for(1.f / *rho_p, 13);   // power of two; |M_ij| <= 1/rho
+  const double v = (i < n && j < n) ? M[i * n + j] * sc : 0.0;
+  const _Float16 h = static_cast<_Float16>(v);
+  Mh[idx] = h;
+  Ml[idx] = static_cast<_Float16>(v - static_cast<double>(h));
+}
+
+__global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
+    int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
+    const unsigned* __restrict__ rowmax, const _Float16* __restrict__ Mh,
+    const _Float16* __restrict__ Ml, float* __restrict__ part) {
+  __shared__ __align__(16) _Float16 Ah[AG_KR][AG_KC + 8];
+  __shared__ __align__(16) _Float16 Al[AG_KR][AG_KC + 8];
+  const int64_t j0 = (int64_t)blockIdx.x * AG_BN;
+  const int split = blockIdx.y;
+  const int64_t k0 = (int64_t)blockIdx.z * AG_KR;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 5, c = lane & 31;
+  const int64_t nch = npad / AG_KC;
+  const int64_t ch0 = (int64_t)split * chunks_per_split;
+  const int64_t ch1 = ch0 + chunks_per_split < nch ? ch0 + chunks_per_split : nch;
+  // staging role: row sr, 32 columns starting at sc0
+  const int sr = tid >> 1, sc0 = (tid & 1) * 32;
+  const float rs = pow2_scale_for(__uint_as_float(rowmax[k0 + sr]), 14);
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  const int64_t jrow0 = j0 + wave * 64 + c;          // M row read by this lane (cb = 0)
+  for (int64_t ch = ch0; ch < ch1; ++ch) {
+    const int64_t i0 = ch * AG_KC;
+    {
+      const float4* src = reinterpret_cast<const float4*>(R + (k0 + sr) * npad + i0 + sc0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float4 v = src[q];
+        const float e[4] = {v.x * rs, v.y * rs, v.z * rs, v.w * rs};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const _Float16 h = static_cast<_Float16>(e[u]);
+          Ah[sr][sc0 + 4 * q + u] = h;
+          Al[sr][sc0 + 4 * q + u] = static_cast<_Float16>(e[u] - static_cast<float>(h));
+        }
+      }
+    }
+    // B fragments: 32 consecutive halves of row (jrow0 + 32 cb) at i0 + 32 g
+    half8 bh[2][4], bl[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int64_t off = (jrow0 + 32 * cb) * npad + i0 + 32 * g;
+      const half8* ph = reinterpret_cast<const half8*>(Mh + off);
+      const half8* pl = reinterpret_cast<const half8*>(Ml + off);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bh[cb][t] = ph[t];
+        bl[cb][t] = pl[t];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][32 * g + 8 * t]);
+        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][32 * g + 8 * t]);
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb][t], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb][t], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb][t], acc[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
+  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * 64;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        out[(int64_t)row * npad + cb * 32 + c] = acc[rb][cb][r];
+      }
+}
+
+// init: z = w_init, u = 0, R = rho (z - u) - c, row maxima of |R|
+__global__ void admm_init2_kernel(int64_t K, int64_t n, int64_t npad, double w_init,
+                                  const double* __restrict__ rho_p, const float* __restrict__ c,
+                                  double* __restrict__ z, double* __restrict__ u,
+                                  float* __restrict__ R, unsigned* __restrict__ rowmax) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= K * n) return;
+  const int64_t k = idx / n, i = idx - k * n;
+  const double rho = *rho_p;
+  z[idx] = w_init;
+  u[idx] = 0.0;
+  const float r = static_cast<float>(rho * (w_init - 0.0) - static_cast<double>(c[idx]));
+  R[k * npad + i] = r;
+  atomicMax(rowmax + k, __float_as_uint(fabsf(r)));   // once per element, once per solve
+}
+
+__device__ __forceinline__ float block_max_256(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float m = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+  return m;
+}
+
+// ADMM step on the summed partial products (nnls.py:152-157), next R and its
+// row maxima.  Grid (ceil(n / 256), K): one row per blockIdx.y, one atomic per
+// block.  rowmax_cur scales this iteration's product, rowmax_next collects the
+// next one, rowmax_clear is reset for the iteration after.
+__global__ __launch_bounds__(256) void admm_update_kernel(
+    int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
+    const double* __restrict__ rho_p, const float* __restrict__ c, double* __restrict__ z,
+    double* __restrict__ u, float* __restrict__ R, const unsigned* __restrict__ rowmax_cur,
+    unsigned* __restrict__ rowmax_next, unsigned* __restrict__ rowmax_clear, double alpha,
+    const int* __restrict__ active, unsigned* __restrict__ delta) {
+  __shared__ float red[4];
+  const int64_t k = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rowmax_clear[k] = 0u;
+    if (k == K - 1)
+      for (int64_t q = K; q < Kp; ++q) rowmax_clear[q] = 0u;
+  }
+  const int64_t o = k * npad + i;
+  float rmax = 0.f, dmax = 0.f;
+  if (!active[k]) {
+    if (i < n) rmax = fabsf(R[o]);
+  } else if (i < n) {
+    const int64_t idx = k * n + i;
+    float sum = 0.f;
+    for (int s = 0; s < S; ++s) sum += part[((int64_t)s * Kp + k) * npad + i];
+    const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
+    const float sR = pow2_scale_for(__uint_as_float(rowmax_cur[k]), 14);
+    const double w = static_cast<double>(sum) / (static_cast<double>(sM) * static_cast<double>(sR));
+    const double rho = *rho_p;
+    const double zo = z[idx], uo = u[idx];
+    const double wh = alpha * w + (1.0 - alpha) * zo;
+    const double zn = fmax(0.0, wh + uo);
+    const double un = (uo + wh) - zn;
+    z[idx] = zn;
+    u[idx] = un;
+    const float r = static_cast<float>(rho * (zn - un) - static_cast<double>(c[idx]));
+    R[o] = r;
+    rmax = fabsf(r);
+    dmax = static_cast<float>(fabs(zn - zo));
+  }
+  rmax = block_max_256(rmax, red);
+  __syncthreads();
+  dmax = block_max_256(dmax, red);
+  if (threadIdx.x == 0) {
+    if (rmax > 0.f) atomicMax(rowmax_next + k, __float_as_uint(rmax));
+    if (dmax > 0.f) atomicMax(delta + k, __float_as_uint(dmax));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -400,21 +497,38 @@ static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct NnlsWs {
   float* Ut;      // n x ldu
-  float* Q;       // n x n (factor / inverse work)
-  float* Mi;      // n x n inverse
-  float* tmp;     // n x NB
+  float* Q;       // n x n FP32 Gram
+  double* Q64;    // n x n FP64, inverted in place
+  double* work64; // n*n/2 work of the recursive inverse
   float* Yf;      // K x ldu
   float* C;       // K x n
-  float* R0;      // K x n
-  float* R1;      // K x n
   double* z;      // K x n
   double* u;      // K x n
   float* rho;     // 1
+  double* rho64;  // 1
   unsigned* delta;  // K
   unsigned* zmax;   // K
   int* active;      // K
+  _Float16* Mh;     // npad x npad
+  _Float16* Ml;     // npad x npad
+  float* Rp;        // Kp x npad
+  float* part;      // S x Kp x npad
+  unsigned* rmax;   // 3 x Kp
+  int64_t npad, Kp;
+  int S;
   int64_t bytes;
 };
+
+// splits of the reduction so that the ADMM product fills two workgroups per CU
+static int admm_splits(int64_t npad, int64_t Kp) {
+  const int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);
+  int64_t S = 512 / (tiles > 0 ? tiles : 1);
+  if (S < 1) S = 1;
+  if (S > 16) S = 16;
+  const int64_t nch = npad / AG_KC;
+  if (S > nch) S = nch;
+  return static_cast<int>(S);
+}
 
 static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   NnlsWs w{};
@@ -428,18 +542,25 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   };
   w.Ut = reinterpret_cast<float*>(take(n * ldu * 4));
   w.Q = reinterpret_cast<float*>(take(n * n * 4));
-  w.Mi = reinterpret_cast<float*>(take(n * n * 4));
-  w.tmp = reinterpret_cast<float*>(take(n * NB * 4));
+  w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
+  w.work64 = reinterpret_cast<double*>(take((n * n / 2 + n) * 8));
   w.Yf = reinterpret_cast<float*>(take(K * ldu * 4));
   w.C = reinterpret_cast<float*>(take(K * n * 4));
-  w.R0 = reinterpret_cast<float*>(take(K * n * 4));
-  w.R1 = reinterpret_cast<float*>(take(K * n * 4));
   w.z = reinterpret_cast<double*>(take(K * n * 8));
   w.u = reinterpret_cast<double*>(take(K * n * 8));
   w.rho = reinterpret_cast<float*>(take(4));
+  w.rho64 = reinterpret_cast<double*>(take(8));
   w.delta = reinterpret_cast<unsigned*>(take(K * 4));
   w.zmax = reinterpret_cast<unsigned*>(take(K * 4));
   w.active = reinterpret_cast<int*>(take(K * 4));
+  w.npad = align_up(n, AG_BN);
+  w.Kp = align_up(K, AG_KR);
+  w.S = admm_splits(w.npad, w.Kp);
+  w.Mh = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
+  w.Ml = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
+  w.Rp = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
+  w.part = reinterpret_cast<float*>(take((int64_t)w.S * w.Kp * w.npad * 4));
+  w.rmax = reinterpret_cast<unsigned*>(take(3 * w.Kp * 4));
   w.bytes = off;
   return w;
 }
@@ -449,47 +570,137 @@ __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
   if (i < n) p[i] = v;
 }
 
-// in-place Cholesky of the n x n matrix A, then A <- L^-1 (lower), then
-// Mi <- (L L^T)^-1
-static int factor_and_invert(hipStream_t st, float* A, float* Mi, float* tmp, int64_t n) {
-  for (int64_t k = 0; k < n; k += NB) {
-    const int jb = static_cast<int>(std::min<int64_t>(NB, n - k));
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, n, k, jb);
-    SMX_LAUNCHED();
-    const int64_t rest = n - k - jb;
-    if (rest > 0) {
-      hipLaunchKernelGGL(trsm_panel_kernel, dim3(blocks_for(rest, 256)), dim3(256), 0, st, A, n, n,
-                         k, jb);
-      SMX_LAUNCHED();
-      // A22 -= L21 L21^T (lower part)
-      if (int e = sgemm<false, true>(st, (int)rest, (int)rest, jb, -1.f, A + (k + jb) * n + k, n,
-                                     A + (k + jb) * n + k, n, 1.f, A + (k + jb) * n + k + jb, n, 0, 1))
-        return e;
-    }
+// dst (cols x rows) = src (rows x cols)^T
+template <class T>
+__global__ void transpose_into_kernel(const T* __restrict__ src, int64_t lds, int64_t rows,
+                                      int64_t cols, T* __restrict__ dst, int64_t ldd) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int64_t r = idx / cols, c = idx - r * cols;
+  dst[c * ldd + r] = src[r * lds + c];
+}
+
+// ---------------------------------------------------------------------------
+// SPD inverse in FP64 by recursive Schur complements, in place on the full
+// matrix:
+//   [A11 A21^T; A21 A22]^-1 with X = A11^-1, T = A21 X, S = A22 - T A21^T:
+//   A22 <- S^-1,  A21 <- -S^-1 T,  A11 <- X - T^T A21,  A12 <- A21^T.
+// 4n^3/3 flops in FP64 MFMA GEMMs; 64 x 64 base blocks by Gauss-Jordan in
+// registers (SPD: no pivoting).  The Schur complements must stay positive
+// definite: in FP32 their error, ~ eps cond(A11) |A21|^2, destroys that at
+// the library's conditioning, in FP64 it is ~1e-9 of the smallest eigenvalue.
+// ---------------------------------------------------------------------------
+constexpr int INV_BASE = 64;
+
+__global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
+  // thread t owns column j = t & 63 for rows h*16 .. h*16+15, h = t >> 6
+  __shared__ double pr[INV_BASE];
+  __shared__ double pc[INV_BASE];
+  const int j = threadIdx.x & (INV_BASE - 1), h = threadIdx.x >> 6;
+  double v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = h * 16 + r;
+    v[r] = (i < nb && j < nb) ? A[(int64_t)i * lda + j] : ((i == j) ? 1.0 : 0.0);
   }
-  hipLaunchKernelGGL(zero_upper_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, A, n);
+  for (int k = 0; k < nb; ++k) {
+    if (h == (k >> 4)) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r == (k & 15)) pr[j] = v[r];
+    }
+    if (j == k) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) pc[h * 16 + r] = v[r];
+    }
+    __syncthreads();
+    const double p = 1.0 / pr[k];
+    const double prj = pr[j];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = h * 16 + r;
+      double x;
+      if (i == k) x = (j == k) ? p : prj * p;
+      else if (j == k) x = -pc[i] * p;
+      else x = v[r] - pc[i] * prj * p;
+      v[r] = x;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = h * 16 + r;
+    if (i < nb && j < nb) A[(int64_t)i * lda + j] = v[r];
+  }
+}
+
+template <class T>
+__global__ void mirror_lower_kernel(T* __restrict__ A, int64_t n, int64_t lda) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx / n, j = idx - i * n;
+  if (j > i) A[i * lda + j] = A[j * lda + i];
+}
+
+static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
+  if (n <= INV_BASE) {
+    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
+    SMX_LAUNCHED();
+    return 0;
+  }
+  const int64_t n1 = ((n / 2 + INV_BASE - 1) / INV_BASE) * INV_BASE;
+  const int64_t n2 = n - n1;
+  double* A11 = A;
+  double* A21 = A + n1 * lda;
+  double* A22 = A + n1 * lda + n1;
+  double* Tm = work;                      // n2 x n1
+  double* deeper = work + n2 * n1;
+  if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
+  if (int e = dgemm<false, false>(st, (int)n2, (int)n1, (int)n1, 1.0, A21, lda, A11, lda, 0.0, Tm, n1))
+    return e;
+  if (int e = dgemm<false, true>(st, (int)n2, (int)n2, (int)n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda))
+    return e;
+  if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
+  if (int e = dgemm<false, false>(st, (int)n2, (int)n1, (int)n2, -1.0, A22, lda, Tm, n1, 0.0, A21, lda))
+    return e;
+  if (int e = dgemm<true, false>(st, (int)n1, (int)n1, (int)n2, -1.0, Tm, n1, A21, lda, 1.0, A11, lda))
+    return e;
+  hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
+                     A21, lda, n2, n1, A + n1, lda);
   SMX_LAUNCHED();
-  // triangular inverse, last block first (LAPACK trtri, lower)
-  const int64_t nblk = (n + NB - 1) / NB;
-  for (int64_t b = nblk - 1; b >= 0; --b) {
-    const int64_t j = b * NB;
-    const int jb = static_cast<int>(std::min<int64_t>(NB, n - j));
-    hipLaunchKernelGGL(trti_diag_kernel, dim3(1), dim3(64), 0, st, A, n, j, jb);
-    SMX_LAUNCHED();
-    const int64_t rest = n - j - jb;
-    if (rest > 0) {
-      // tmp = Z22 * L21   (Z22 lower-triangular, already inverted)
-      if (int e = sgemm<false, false>(st, (int)rest, jb, (int)rest, 1.f, A + (j + jb) * n + j + jb, n,
-                                      A + (j + jb) * n + j, n, 0.f, tmp, NB, 1, 0))
-        return e;
-      // Z21 = -tmp * Z11
-      if (int e = sgemm<false, false>(st, (int)rest, jb, jb, -1.f, tmp, NB, A + j * n + j, n, 0.f,
-                                      A + (j + jb) * n + j, n, 0, 0))
-        return e;
-    }
+  return 0;
+}
+
+// Q (FP32, full) -> FP64 with Q += reg I, rho = tr(Q)/n, Q += rho I (nnls.py:115-121)
+__global__ void widen_kernel(const float* __restrict__ Q, int64_t nn, double* __restrict__ D) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx < nn) D[idx] = static_cast<double>(Q[idx]);
+}
+
+__global__ __launch_bounds__(1024) void regularise64_kernel(double* __restrict__ Q, int64_t n,
+                                                            double reg, double* __restrict__ rho_out,
+                                                            float* __restrict__ rho_f) {
+  __shared__ double red[16];
+  __shared__ double rho_s;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double d = Q[i * n + i] + reg;
+    Q[i * n + i] = d;
+    acc += d;
   }
-  // Mi = Z^T Z, using k >= max(i, j)
-  return sgemm<true, false>(st, (int)n, (int)n, (int)n, 1.f, A, n, A, n, 0.f, Mi, n, 2, 0);
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    rho_s = t / static_cast<double>(n);
+    *rho_out = rho_s;
+    *rho_f = static_cast<float>(rho_s);
+  }
+  __syncthreads();
+  const double rho = rho_s;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Q[i * n + i] = Q[i * n + i] + rho;
 }
 
 }  // namespace smx
@@ -512,45 +723,70 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipStream_t st = as_stream(stream);
   NnlsWs w = carve(workspace, n, m, K);
   const int64_t ldu = align_up(m, 16);
+  stage_begin(ST_GRAM, st);
   hipLaunchKernelGGL(build_ut_kernel, dim3(blocks_for(n * ldu, 256)), dim3(256), 0, st, lib, ld_lib,
                      n, m, ldu, row_scale, w.Ut);
   SMX_LAUNCHED();
-  // Q = U^T U (lower only), mirrored later by Z^T Z
-  if (int e = sgemm<false, true>(st, (int)n, (int)n, (int)ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n,
-                                 0, 1))
+  // Q = U^T U: lower half on MFMA (FP32), mirrored, widened to FP64
+  if (int e = gemm<false, true>(st, (int)n, (int)n, (int)ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1))
     return e;
-  hipLaunchKernelGGL(regularise_kernel, dim3(1), dim3(1024), 0, st, w.Q, n,
-                     static_cast<float>(lambda_reg / static_cast<double>(n)), w.rho);
+  hipLaunchKernelGGL(mirror_lower_kernel<float>, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n, n);
   SMX_LAUNCHED();
-  if (int e = factor_and_invert(st, w.Q, w.Mi, w.tmp, n)) return e;
+  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
+                     lambda_reg / static_cast<double>(n), w.rho64, w.rho);
+  SMX_LAUNCHED();
+  stage_end(ST_GRAM, st, 5);
+  // Q + rho I <- (Q + rho I)^-1 in place, FP64
+  stage_begin(ST_FACTOR, st);
+  if (int e = inv_spd_rec(st, w.Q64, n, n, w.work64)) return e;
+  stage_end(ST_FACTOR, st, 1);
+  stage_begin(ST_RHS, st);
   // C = -Y U
   hipLaunchKernelGGL(to_float_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m, ldu,
                      w.Yf);
   SMX_LAUNCHED();
-  if (int e = sgemm<false, true>(st, (int)K, (int)n, (int)ldu, -1.f, w.Yf, ldu, w.Ut, ldu, 0.f, w.C,
-                                 n, 0, 0))
+  if (int e = gemm<false, true>(st, (int)K, (int)n, (int)ldu, -1.f, w.Yf, ldu, w.Ut, ldu, 0.f, w.C, n))
     return e;
   const int64_t tot = K * n;
-  hipLaunchKernelGGL(admm_init_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w_init,
-                     w.rho, w.C, w.z, w.u, w.R0);
+  const int64_t npad = w.npad, Kp = w.Kp;
+  const int S = w.S;
+  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad, 256)), dim3(256), 0, st, w.Q64, n,
+                     npad, w.rho, w.Mh, w.Ml);
   SMX_LAUNCHED();
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st,
+                     reinterpret_cast<unsigned*>(w.Rp), Kp * npad, 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(3 * Kp, 256)), dim3(256), 0, st, w.rmax,
+                     3 * Kp, 0u);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(admm_init2_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, K, n, npad,
+                     w_init, w.rho64, w.C, w.z, w.u, w.Rp, w.rmax);
+  SMX_LAUNCHED();
+  stage_end(ST_RHS, st, 5);
   std::vector<int> act(K, 1);
   SMX_HIP(hipMemcpyAsync(w.active, act.data(), K * sizeof(int), hipMemcpyHostToDevice, st));
   const int check = 25;
   std::vector<unsigned> dl(K), zm(K);
-  float* Rc = w.R0;
-  float* Rn = w.R1;
   int done = 0;
-  dim3 grid((unsigned)((n + BN - 1) / BN), (unsigned)((K + BM - 1) / BM));
+  const int64_t nch = npad / AG_KC;
+  const int cps = static_cast<int>((nch + S - 1) / S);
+  dim3 ggrid((unsigned)(npad / AG_BN), (unsigned)S, (unsigned)(Kp / AG_KR));
+  stage_begin(ST_ADMM, st);
   for (int it = 0; it < maxiter; ++it) {
     const bool chk = tol > 0.0 && ((it + 1) % check == 0);
-    if (chk) {
+    if (chk)
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.delta, K, 0u);
-    }
-    hipLaunchKernelGGL(admm_step_kernel, grid, dim3(256), 0, st, (int)K, (int)n, Rc, w.Mi, w.rho,
-                       w.C, w.z, w.u, Rn, 1.6, w.active, w.delta);
+    unsigned* rcur = w.rmax + (it % 3) * Kp;
+    unsigned* rnext = w.rmax + ((it + 1) % 3) * Kp;
+    unsigned* rclear = w.rmax + ((it + 2) % 3) * Kp;
+    hipLaunchKernelGGL(admm_gemm_kernel, ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rp, rcur, w.Mh,
+                       w.Ml, w.part);
     SMX_LAUNCHED();
-    std::swap(Rc, Rn);
+    hipLaunchKernelGGL(admm_update_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
+                       K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext, rclear,
+                       1.6, w.active, w.delta);
+    SMX_LAUNCHED();
     done = it + 1;
     if (chk) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.zmax, K, 0u);
@@ -570,19 +806,49 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
       if (!any) break;
     }
   }
+  stage_end(ST_ADMM, st, done);
   hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w.z, W);
   SMX_LAUNCHED();
   if (iters_done) *iters_done = done;
   return 0;
 }
 
+int smx_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, float alpha,
+                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+                 int64_t ldc, void* stream) {
+  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0, "smx_gemm_f32: bad arguments");
+  hipStream_t st = as_stream(stream);
+  SMX_REQUIRE(!(trans_a && trans_b), "smx_gemm_f32: both transposed is not supported");
+  if (trans_a) return gemm<true, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (trans_b) return gemm<false, true>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+  return gemm<false, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int smx_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                 int64_t ldc, void* stream) {
+  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0, "smx_gemm_f64: bad arguments");
+  SMX_REQUIRE(!(trans_a && trans_b), "smx_gemm_f64: both transposed is not supported");
+  hipStream_t st = as_stream(stream);
+  if (trans_a) return dgemm<true, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (trans_b) return dgemm<false, true>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+  return dgemm<false, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream) {
+  SMX_REQUIRE(A && work && n > 0 && n < ((int64_t)1 << 31), "smx_spd_inverse_f64: bad arguments");
+  return inv_spd_rec(as_stream(stream), A, n, n, work);
+}
+
 int smx_model_maps(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const double* W,
                    int64_t K, double* maps, void* stream) {
   SMX_REQUIRE(lib && W && maps && n > 0 && m > 0 && K > 0 && ld_lib >= m, "smx_model_maps: bad arguments");
   dim3 grid((unsigned)((m + 63) / 64), (unsigned)((K + 15) / 16));
+  stage_begin(ST_MAPS, as_stream(stream));
   hipLaunchKernelGGL(maps_kernel, grid, dim3(256), 0, as_stream(stream), (int)K, n, m, W, lib, ld_lib,
                      maps);
   SMX_LAUNCHED();
+  stage_end(ST_MAPS, as_stream(stream), 1);
   return 0;
 }
 

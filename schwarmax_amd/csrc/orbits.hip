@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "potential.hpp"
 #include "binning.hpp"
+#include "timing.hpp"
 
 namespace smx {
 
@@ -40,14 +41,12 @@ __global__ void potential_eval_kernel(smx_potential p, const double* __restrict_
   const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
   if (acc) {
     double ax, ay, az;
-    const double ph = accel(p, x, y, z, ax, ay, az);
+    accel_fast(p, x, y, z, ax, ay, az);
     acc[3 * i] = ax;
     acc[3 * i + 1] = ay;
     acc[3 * i + 2] = az;
-    if (phi) phi[i] = ph;
-  } else if (phi) {
-    phi[i] = phi_total<double>(p, x, y, z);
   }
+  if (phi) phi[i] = phi_total(p, x, y, z);
 }
 
 __global__ void density_eval_kernel(smx_potential p, const double* __restrict__ xyz, int64_t n,
@@ -75,7 +74,7 @@ __device__ __forceinline__ double lin_pt(double lo, double hi, double step, int 
 
 __device__ __forceinline__ double dphi_dz(const smx_potential& p, double x, double y, double z) {
   const double d = 5e-3;
-  return (phi_total<double>(p, x, y, z + d) - phi_total<double>(p, x, y, z - d)) / (2.0 * d);
+  return (phi_total(p, x, y, z + d) - phi_total(p, x, y, z - d)) / (2.0 * d);
 }
 
 // integral_{|z*|}^{10} rho(x, y, z') dPhi/dz(x, y, z') dz' by the trapezoid
@@ -120,7 +119,7 @@ __global__ __launch_bounds__(256) void initial_conditions_kernel(
   const double sR = sqrt(sR2);
   const double dnu = b * (Ip - Im) / (2.0 * dR);
   const double d = 5e-3;
-  const double dphidR = (phi_total<double>(p, Rs + d, 0.0, zs) - phi_total<double>(p, Rs - d, 0.0, zs)) / (2.0 * d);
+  const double dphidR = (phi_total(p, Rs + d, 0.0, zs) - phi_total(p, Rs - d, 0.0, zs)) / (2.0 * d);
   const double vtot2 = (sR2 + (Rs / nu) * dnu) + Rs * dphidR;
   const double sphi = sR;
   double vmean = sqrt(nmax(vtot2 - sphi * sphi, 0.0));
@@ -136,7 +135,7 @@ __global__ __launch_bounds__(256) void initial_conditions_kernel(
   const double R = sqrt(wn[0] * wn[0] + wn[1] * wn[1]);
   const double dRt = 1e-3;
   const double Rsafe = fmax(fabs(R), 2.0 * dRt);
-  const double dd = (phi_total<double>(p, Rsafe + dRt, 0.0, zs) - phi_total<double>(p, Rsafe - dRt, 0.0, zs)) / (2.0 * dRt);
+  const double dd = (phi_total(p, Rsafe + dRt, 0.0, zs) - phi_total(p, Rsafe - dRt, 0.0, zs)) / (2.0 * dRt);
   const double vc = sqrt(nmax(Rsafe * dd, 0.0));
   const double om = sqrt(nmax(fabs(dd) / Rsafe, 1e-20));
   const double T_orb = (2.0 * kPi) / om;
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(256) void initial_conditions_kernel(
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void deriv(const smx_potential& p, double Om, const double* y, double* k) {
   double ax, ay, az;
-  accel(p, y[0], y[1], y[2], ax, ay, az);
+  accel_fast(p, y[0], y[1], y[2], ax, ay, az);
   k[0] = y[3] + Om * y[1];
   k[1] = y[4] - Om * y[0];
   k[2] = y[5];
@@ -167,30 +166,33 @@ __device__ __forceinline__ void deriv(const smx_potential& p, double Om, const d
 
 __device__ __forceinline__ double jacobi(const smx_potential& p, double Om, const double* y) {
   const double ek = 0.5 * ((y[3] * y[3] + y[4] * y[4]) + y[5] * y[5]);
-  return (ek + phi_total<double>(p, y[0], y[1], y[2])) - Om * (y[0] * y[4] - y[1] * y[3]);
+  return (ek + phi_total(p, y[0], y[1], y[2])) - Om * (y[0] * y[4] - y[1] * y[3]);
 }
 
 // One attempt.  On return y holds y_out, k1 the FSAL slope, dt the next step;
-// returns the step length used when accepted, 0 otherwise.
+// returns the step length used when accepted, 0 otherwise.  The stage
+// vectors are folded as soon as they are complete (the error sum's first
+// three terms before k4 is formed), keeping the reference's summation order.
 __device__ __forceinline__ double bs3_attempt(const smx_potential& p, const smx_schedule& s,
                                               double& t, double* y, double* k1, double& dt) {
   const double Om = s.Omega;
-  double yt[6], k2[6], k3[6], k4[6], yn[6];
+  double yt[6], k2[6], kk[6], yn[6], e3[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.5 * dt) * k1[j];
   deriv(p, Om, yt, k2);
 #pragma unroll
   for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.75 * dt) * k2[j];
-  deriv(p, Om, yt, k3);
+  deriv(p, Om, yt, kk);                                  // k3
 #pragma unroll
-  for (int j = 0; j < 6; ++j)
-    yn[j] = y[j] + dt * ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * k3[j]);
-  deriv(p, Om, yn, k4);
+  for (int j = 0; j < 6; ++j) {
+    yn[j] = y[j] + dt * ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * kk[j]);
+    e3[j] = (-5.0 / 72.0 * k1[j] + 1.0 / 12.0 * k2[j]) + 1.0 / 9.0 * kk[j];
+  }
+  deriv(p, Om, yn, kk);                                  // k4
   double es = 0.0;
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const double e = dt * (((-5.0 / 72.0 * k1[j] + 1.0 / 12.0 * k2[j]) + 1.0 / 9.0 * k3[j]) +
-                           -1.0 / 8.0 * k4[j]);
+    const double e = dt * (e3[j] + -1.0 / 8.0 * kk[j]);
     const double sc = s.atol + s.rtol * nmax(fabs(y[j]), fabs(yn[j]));
     const double q = e / sc;
     es += q * q;
@@ -203,7 +205,7 @@ __device__ __forceinline__ double bs3_attempt(const smx_potential& p, const smx_
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       y[j] = yn[j];
-      k1[j] = k4[j];
+      k1[j] = kk[j];
     }
   }
   double sf = 0.9 * pow(nmax(err, 1e-10), -1.0 / 3.0);
@@ -502,10 +504,12 @@ int smx_initial_conditions(const smx_potential* p, const double* w0, const doubl
   SMX_REQUIRE(n >= 0 && r >= 1 && r <= r_total, "smx_initial_conditions: bad sizes");
   if (n == 0) return 0;
   const int waves_per_block = 4;
+  stage_begin(ST_ICS, as_stream(stream));
   hipLaunchKernelGGL(initial_conditions_kernel, dim3(blocks_for(n, waves_per_block)),
                      dim3(64 * waves_per_block), 0, as_stream(stream), *p, w0, normals, uniforms,
                      n, r, r_total, N_step_per_orb, N_dyn, wb, T_total, dt0);
   SMX_LAUNCHED();
+  stage_end(ST_ICS, as_stream(stream), 1);
   return 0;
 }
 
@@ -538,14 +542,18 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
   double* ws = static_cast<double*>(workspace);
   hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, ws, words, 0.0);
   SMX_LAUNCHED();
+  stage_begin(ST_INTEGRATE, st);
   hipLaunchKernelGGL(integrate_library_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p,
                      *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp, ws, valid, T_int,
                      t_final);
   SMX_LAUNCHED();
+  stage_end(ST_INTEGRATE, st, 1);
   const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
+  stage_begin(ST_FINALIZE, st);
   hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * rows, 256)), dim3(256), 0, st,
                      ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 0, lib);
   SMX_LAUNCHED();
+  stage_end(ST_FINALIZE, st, 1);
   return 0;
 }
 
@@ -607,9 +615,11 @@ int smx_density_target(const smx_potential* p, const double* R_grid, const doubl
   SMX_REQUIRE(p && R_grid && z_grid && phi_grid && sample && out && P >= 0 && S > 0,
               "smx_density_target: bad arguments");
   if (P == 0) return 0;
+  stage_begin(ST_DENSITY, as_stream(stream));
   hipLaunchKernelGGL(density_target_kernel, dim3((unsigned)P), dim3(256), 0, as_stream(stream), *p,
                      R_grid, z_grid, phi_grid, P, dR, dz, dphi, sample, S, out);
   SMX_LAUNCHED();
+  stage_end(ST_DENSITY, as_stream(stream), 1);
   return 0;
 }
 

@@ -87,7 +87,9 @@ def cpu_baseline(config, seed, n_int_cols=64, n_nnls=1000):
                          dt0.reshape(-1), -pb['Omega_bar'], s['atol'], s['rtol'], s['dt_min'],
                          s['dt_max'], og, chunk_size=s['chunk'])
     t_int = time.time() - t0
-    # NNLS sample: n_nnls columns, the workload's m rows, K rhs, T iterations
+    # NNLS sample: n_nnls columns, the workload's m rows, K rhs, T iterations.
+    # Each part is scaled by its own order in n: Gram U^T U ~ n^2 m, Cholesky
+    # ~ n^3, one ADMM iteration (two triangular solves per rhs) ~ n^2 K.
     rng = np.random.default_rng(1)
     m = int(np.prod(Rn)) + 5 * int(dd['total_bins'])
     K, T = cfg['K'], cfg['T']
@@ -95,24 +97,36 @@ def cpu_baseline(config, seed, n_int_cols=64, n_nnls=1000):
     Y = rng.random((K, m))
     with threadpoolctl.threadpool_limits(threads):
         t0 = time.time()
+        Q = U.T @ U
+        t_gram = time.time() - t0
+        t0 = time.time()
         _, rho, Lc = ON.gram_factor(U, 1.0)
-        t_fac = time.time() - t0
+        t_fac = max(time.time() - t0 - t_gram, 0.0)
         t0 = time.time()
         ON.admm(Lc, rho, -(Y @ U), 1.0, T)
         t_admm = time.time() - t0
+    del Q
     scale_n = n_full / n_int_cols
     r = n_full / n_nnls
-    # integration / ICs linear in n; Gram + factor ~ n^2 m + n^3 (use n^3 bound
-    # of the slower of the two as r^3 on the measured factor time, r^2 for the
-    # Gram share is folded in conservatively by using r^2.5); ADMM ~ n^2
-    est = (t_ics + t_int) * scale_n + t_fac * r ** 2.5 + t_admm * r ** 2
+    est = (t_ics + t_int) * scale_n + t_gram * r ** 2 + t_fac * r ** 3 + t_admm * r ** 2
     sample = (f"oracle on {n_int_cols} of {n_full} columns x4 realizations x{s['N_max']} steps "
-              f"(ICs {t_ics:.1f}s + integrate/bin {t_int:.1f}s, x{scale_n:.0f} linear) and NNLS "
-              f"at n={n_nnls}, m={m}, K={K}, T={T} (factor {t_fac:.1f}s x{r:.0f}^2.5, ADMM "
-              f"{t_admm:.1f}s x{r:.0f}^2); extrapolated {est:.0f} s/eval; NumPy elementwise on 1 "
-              f"thread, BLAS on {threads}")
+              f"(ICs {t_ics:.1f}s + integrate/bin {t_int:.1f}s, x{scale_n:.0f}) and NNLS at "
+              f"n={n_nnls}, m={m}, K={K}, T={T} (Gram {t_gram:.2f}s x{r:.0f}^2, Cholesky "
+              f"{t_fac:.2f}s x{r:.0f}^3, ADMM {t_admm:.1f}s x{r:.0f}^2); extrapolated "
+              f"{est:.0f} s/eval; NumPy elementwise on 1 thread, BLAS/LAPACK on {threads}")
     return {'value': 1.0 / est, 'unit': 'model evals/s', 'cores': threads, 'kind': 'port',
             'sample': sample}
+
+
+def traffic_per_launch(config):
+    """HBM bytes per ADMM-product launch from the committed rocprofv3 --pmc
+    summary for this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    same counter's reading on a copy of known size), or None."""
+    path = os.path.join(ROOT, 'profiles', f'admm_gemm_kernel.{config}.traffic.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get('bytes_per_launch')
 
 
 def main():
@@ -135,10 +149,12 @@ def main():
     assert torch.cuda.is_available(), 'needs a GPU'
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
-        a = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+        # one elementwise kernel per pass (b = a * 1): reads and writes nbytes;
+        # no fill kernel, so the counters see only these dispatches
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda')
         b = torch.empty_like(a)
         for _ in range(5):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes': nbytes}))
         return
@@ -174,30 +190,34 @@ def main():
     log('logL', [float(v) for v in vals], f'{dt / args.steps:.3f} s/eval')
     log('stages (ms per eval):', {k: round(v[0] / args.steps, 2) for k, v in stages.items()})
 
-    # roofline of the dominant kernel
+    # Roofline of the dominant kernel: whichever of the ADMM product (HBM
+    # bound: both fp16 planes of M once, R in, the product out) and the
+    # integrator (FP64 arithmetic) takes the larger share of the step.
     ph, pb = OM.theta_to_params(theta)
     _, f_acc, _ = opcount.force_flops(OM.baryon_params(pb), ph)
     N = sched['N_max']
     n_int = n * sched['n_realizations']
     int_ms = stages['integrate'][0] / max(stages['integrate'][1], 1)
     int_flops = n_int * N * (3 * f_acc + opcount.F_STEP)
-    admm_ms = stages['admm'][0] / max(stages['admm'][1], 1)
-    admm_flops = 2.0 * n * n * K
-    if stages['integrate'][0] >= stages['admm'][0]:
+    prod_ms = stages['admm_product'][0] / max(stages['admm_product'][1], 1)
+    prod_bytes = 4.0 * n * n + 4.0 * K * n + 4.0 * K * n
+    if stages['integrate'][0] >= stages['admm_product'][0]:
         ach = int_flops / (int_ms * 1e-3) / 1e12
-        roof = {'bound': 'mfma', 'kernel': 'integrate_library_kernel (FP64 VALU)', 'achieved': ach,
-                'peak': PEAK['fp64'], 'unit': 'TFLOP/s', 'frac': ach / PEAK['fp64'], 'traffic': None,
-                'flops_per_launch': int_flops, 'ms_per_launch': int_ms}
+        roof = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK['fp64'], 'unit': 'TFLOP/s',
+                'frac': ach / PEAK['fp64'], 'traffic': None,
+                'kernel': 'integrate_library_kernel (FP64)', 'per_launch': int_flops,
+                'ms_per_launch': int_ms}
     else:
-        ach = admm_flops / (admm_ms * 1e-3) / 1e12
-        roof = {'bound': 'mfma', 'kernel': 'admm_step_kernel (FP32)', 'achieved': ach,
-                'peak': PEAK['fp32'], 'unit': 'TFLOP/s', 'frac': ach / PEAK['fp32'], 'traffic': None,
-                'flops_per_launch': admm_flops, 'ms_per_launch': admm_ms}
+        ach = prod_bytes / (prod_ms * 1e-3) / 1e9
+        roof = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK['hbm'], 'unit': 'GB/s',
+                'frac': ach / PEAK['hbm'], 'traffic': traffic_per_launch(args.config),
+                'kernel': 'admm_gemm_kernel', 'per_launch': prod_bytes,
+                'ms_per_launch': prod_ms, 'launches': stages['admm_product'][1]}
     out = {
         'metric': 'model evals/s @20k orbits (plain)', 'value': args.steps / dt,
         'unit': 'model evals/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': 1e3 * dt / args.steps, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f64 integration / f32 NNLS', 'data': 'synthetic',
+        'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic',
         'config': {'workload': args.config, 'orbits': n, 'realizations': sched['n_realizations'],
                    'bs3_attempts': N, 'apertures': B, 'rzphi_cells': P, 'bootstrap_rhs': K,
                    'admm_iters': T, 'nnls_tol': args.nnls_tol},

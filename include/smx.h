@@ -70,7 +70,8 @@ int smx_version(void);
 /* Per-stage timing with HIP events on the launch stream (no counterpart in
  * the reference; used by the benchmark).  Stages, in order: initial
  * conditions, integration, finalize, Gram, factor + inverse, right-hand sides,
- * ADMM loop, model maps, density target.  enable != 0 turns recording on and
+ * ADMM loop, model maps, density target, and the ADMM product kernel alone
+ * (one record per launch, nested inside the ADMM loop).  enable != 0 turns recording on and
  * clears previous records; read fills ms[] and launches[] (kernel launches per
  * stage) summed over the records since, then clears them. */
 int smx_timing_enable(int enable);
@@ -155,17 +156,14 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
 
 /* ---- dense building blocks (no reference counterpart: the reference calls
  * XLA's dot / cholesky / cho_solve; exposed for tests) ------------------- */
-/* C = alpha op(A) op(B) + beta C in FP32 on MFMA; op(A) is (M,K), op(B) (K,N);
- * trans_a: A stored (K, M); trans_b: B stored (N, K); not both. */
-int smx_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, float alpha,
-                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
-                 int64_t ldc, void* stream);
-/* Same in FP64 on MFMA. */
-int smx_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, double alpha,
-                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
-                 int64_t ldc, void* stream);
+/* C = alpha A B^T + beta C on MFMA, A (M,K) and B (N,K) row-major (k
+ * contiguous), C (M,N); FP32 and FP64. */
+int smx_gemm_nt_f32(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
+                    const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream);
+int smx_gemm_nt_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                    const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream);
 /* In-place inverse of a full symmetric positive definite n x n FP64 matrix
- * (recursive Schur complements); work holds n*n/2 + n doubles. */
+ * (recursive Schur complements); work holds n*n + n doubles. */
 int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream);
 
 /* ---- model maps and likelihood (model.py:14-68, likelihood.py:199-244) -- */

@@ -78,10 +78,9 @@ _SIGS = [
     ('smx_nnls_workspace_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_admm', ctypes.c_int,
      [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
-    ('smx_gemm_f32', ctypes.c_int, [_I32, _I32, _I64, _I64, _I64, ctypes.c_float, _P, _I64, _P,
-                                    _I64, ctypes.c_float, _P, _I64, _P]),
-    ('smx_gemm_f64', ctypes.c_int, [_I32, _I32, _I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P,
-                                    _I64, _P]),
+    ('smx_gemm_nt_f32', ctypes.c_int, [_I64, _I64, _I64, ctypes.c_float, _P, _I64, _P, _I64,
+                                       ctypes.c_float, _P, _I64, _P]),
+    ('smx_gemm_nt_f64', ctypes.c_int, [_I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P, _I64, _P]),
     ('smx_spd_inverse_f64', ctypes.c_int, [_P, _I64, _P, _P]),
     ('smx_model_maps', ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _P, _P]),
 ]
@@ -146,7 +145,8 @@ def dev(a, dtype=torch.float64, device='cuda'):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
 
-STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps', 'density')
+STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps', 'density',
+          'admm_product')
 
 
 def timing_enable(on=True):

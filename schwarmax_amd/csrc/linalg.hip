@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <vector>
 #include <cstring>
+#include <cstdlib>
 #include <algorithm>
 
 #include "common.hpp"
@@ -35,177 +36,179 @@ namespace smx {
 // lower_only: skip tiles strictly above the diagonal.
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
-// FP32 GEMM on MFMA (v_mfma_f32_32x32x2f32), 128 x 128 tiles, 4 waves each
-// owning 64 x 64 of the output.
+// NT GEMM on MFMA:  C[i][j] = alpha * sum_k A[i*lda + k] B[j*ldb + k] + beta * C[i][j]
+// (both operands k-contiguous).  128 x 128 output tiles, 4 waves of 64 x 64,
+// K in tiles of 32 staged through LDS k-major; the next tile is loaded into
+// registers while the current one is multiplied.
+//   FP64: v_mfma_f64_16x16x4f64, 4 x 4 blocks per wave; lane (g = l>>4, c = l&15),
+//         accumulator element r -> row g + 4r, column c.
+//   FP32: v_mfma_f32_32x32x2f32, 2 x 2 blocks per wave; lane (g = l>>5, c = l&31),
+//         accumulator element r -> row 8(r/4) + 4g + r%4, column c.
+// lower_only skips output tiles strictly above the diagonal and stores only j <= i.
 // ---------------------------------------------------------------------------
+constexpr int NT_TM = 128, NT_TN = 128, NT_TK = 32;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16m __attribute__((ext_vector_type(16)));
 
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256, 2) void gemm_mfma_kernel(int M, int N, int K, float alpha,
-                                                           const float* __restrict__ A, int64_t lda,
-                                                           const float* __restrict__ B, int64_t ldb,
-                                                           float beta, float* __restrict__ C,
-                                                           int64_t ldc, int lower_only) {
-  constexpr int TM = 128, TN = 128, TK = 32;
-  __shared__ float As[TK][TM + 4];
-  __shared__ float Bs[TK][TN + 4];
-  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
-  if (lower_only && j0 > i0 + TM - 1) return;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = lane >> 5, c = lane & 31;
+template <class T> struct NtMfma;
+template <> struct NtMfma<double> {
+  static constexpr int BLK = 16, KS = 4, NB = 4;       // block edge, k per MFMA, blocks per wave edge
+  typedef f64x4 acc_t;
+  static constexpr int ACC = 4;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  __device__ static int col(int lane) { return lane & 15; }
+  __device__ static int kof(int lane) { return lane >> 4; }
+};
+template <> struct NtMfma<float> {
+  static constexpr int BLK = 32, KS = 2, NB = 2;
+  typedef f32x16m acc_t;
+  static constexpr int ACC = 16;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+  __device__ static int col(int lane) { return lane & 31; }
+  __device__ static int kof(int lane) { return lane >> 5; }
+};
+
+// One operand tile (128 rows x 32 k) per thread: EPT elements as VEC-wide vectors.
+template <class T, bool VEC>
+struct NtLoader {
+  static constexpr int V = VEC ? 16 / (int)sizeof(T) : 1;  // elements per load
+  static constexpr int TPR = NT_TK / V;                     // threads per row
+  static constexpr int RPP = 256 / TPR;                     // rows per pass
+  static constexpr int PASSES = NT_TM / RPP;
+  T v[PASSES][V];
+  __device__ void load(const T* __restrict__ P, int64_t ld, int rows, int r0, int K, int k0) {
+    const int t = threadIdx.x;
+    const int kk = (t % TPR) * V;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+      const int rr = t / TPR + p * RPP;
+      const int gr = r0 + rr;
+      const int gk = k0 + kk;
+      if (VEC) {
+        if (gr < rows && gk < K) {
+          const T* src = P + (int64_t)gr * ld + gk;
+          if (sizeof(T) == 8) {
+            const double2 x = *reinterpret_cast<const double2*>(src);
+            v[p][0] = (T)x.x;
+            v[p][1] = (T)x.y;
+          } else {
+            const float4 x = *reinterpret_cast<const float4*>(src);
+            v[p][0] = (T)x.x;
+            v[p][1] = (T)x.y;
+            if (V > 2) {
+              v[p][V > 2 ? 2 : 0] = (T)x.z;
+              v[p][V > 3 ? 3 : 0] = (T)x.w;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[p][e] = (T)0;
+        }
+      } else {
+        v[p][0] = (gr < rows && gk < K) ? P[(int64_t)gr * ld + gk] : (T)0;
+      }
+    }
+  }
+  template <int PITCH>
+  __device__ void store(T (*S)[PITCH]) const {
+    const int t = threadIdx.x;
+    const int kk = (t % TPR) * V;
+#pragma unroll
+    for (int p = 0; p < PASSES; ++p) {
+      const int rr = t / TPR + p * RPP;
+#pragma unroll
+      for (int e = 0; e < V; ++e) S[kk + e][rr] = v[p][e];
+    }
+  }
+};
+
+template <class T, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_nt_kernel(int M, int N, int K, T alpha,
+                                                         const T* __restrict__ A, int64_t lda,
+                                                         const T* __restrict__ B, int64_t ldb, T beta,
+                                                         T* __restrict__ C, int64_t ldc, int lower_only) {
+  typedef NtMfma<T> MM;
+  constexpr int PITCH = NT_TM + (sizeof(T) == 8 ? 2 : 4);
+  __shared__ T As[NT_TK][PITCH];
+  __shared__ T Bs[NT_TK][PITCH];
+  const int i0 = blockIdx.y * NT_TM, j0 = blockIdx.x * NT_TN;
+  if (lower_only && j0 > i0 + NT_TM - 1) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  f32x16m acc[2][2];
+  const int g = MM::kof(lane), c = MM::col(lane);
+  typename MM::acc_t acc[MM::NB][MM::NB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MM::NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < MM::NB; ++b)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += TK) {
-#pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      const int e = tid + 256 * l;
-      int ii, kk;
-      if (TA) { ii = e & (TM - 1); kk = e >> 7; } else { kk = e & (TK - 1); ii = e >> 5; }
-      const int gi = i0 + ii, gk = k0 + kk;
-      float v = 0.f;
-      if (gi < M && gk < K) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
-      As[kk][ii] = v;
+      for (int r = 0; r < MM::ACC; ++r) acc[a][b][r] = (T)0;
+  NtLoader<T, VEC> la, lb;
+  la.load(A, lda, M, i0, K, 0);
+  lb.load(B, ldb, N, j0, K, 0);
+  la.store(As);
+  lb.store(Bs);
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += NT_TK) {
+    const bool more = k0 + NT_TK < K;
+    if (more) {
+      la.load(A, lda, M, i0, K, k0 + NT_TK);
+      lb.load(B, ldb, N, j0, K, k0 + NT_TK);
     }
 #pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      const int e = tid + 256 * l;
-      int jj, kk;
-      if (TB) { kk = e & (TK - 1); jj = e >> 5; } else { jj = e & (TN - 1); kk = e >> 7; }
-      const int gj = j0 + jj, gk = k0 + kk;
-      float v = 0.f;
-      if (gj < N && gk < K) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
-      Bs[kk][jj] = v;
+    for (int kk = 0; kk < NT_TK; kk += MM::KS) {
+      T av[MM::NB], bv[MM::NB];
+#pragma unroll
+      for (int r = 0; r < MM::NB; ++r) av[r] = As[kk + g][wi + r * MM::BLK + c];
+#pragma unroll
+      for (int r = 0; r < MM::NB; ++r) bv[r] = Bs[kk + g][wj + r * MM::BLK + c];
+#pragma unroll
+      for (int rb = 0; rb < MM::NB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < MM::NB; ++cb) acc[rb][cb] = MM::mma(av[rb], bv[cb], acc[rb][cb]);
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 2) {
-      const float a0 = As[kk + g][wi + c], a1 = As[kk + g][wi + 32 + c];
-      const float b0 = Bs[kk + g][wj + c], b1 = Bs[kk + g][wj + 32 + c];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+    if (more) {
+      la.store(As);
+      lb.store(Bs);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int rb = 0; rb < MM::NB; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < MM::NB; ++cb)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gi = i0 + wi + rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        const int gj = j0 + wj + cb * 32 + c;
+      for (int r = 0; r < MM::ACC; ++r) {
+        const int gi = i0 + wi + rb * MM::BLK + MM::row(lane, r);
+        const int gj = j0 + wj + cb * MM::BLK + c;
         if (gi >= M || gj >= N) continue;
         if (lower_only && gj > gi) continue;
-        float* p = C + (int64_t)gi * ldc + gj;
-        *p = (beta == 0.f) ? alpha * acc[rb][cb][r] : alpha * acc[rb][cb][r] + beta * (*p);
+        T* p = C + (int64_t)gi * ldc + gj;
+        *p = (beta == (T)0) ? alpha * acc[rb][cb][r] : alpha * acc[rb][cb][r] + beta * (*p);
       }
 }
 
-template <bool TA, bool TB>
-static int gemm(hipStream_t st, int M, int N, int K, float alpha, const float* A, int64_t lda,
-                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, int lower_only = 0) {
+template <class T>
+static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
+                   const T* B, int64_t ldb, T beta, T* C, int64_t ldc, int lower_only = 0) {
   if (M <= 0 || N <= 0) return 0;
-  dim3 grid((N + 127) / 128, (M + 127) / 128);
-  hipLaunchKernelGGL((gemm_mfma_kernel<TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B,
-                     ldb, beta, C, ldc, lower_only);
-  SMX_LAUNCHED();
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
-// FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64), 128 x 128 tiles, 4 waves each
-// owning 64 x 64 (4 x 4 blocks of 16 x 16); same operand conventions.
-// ---------------------------------------------------------------------------
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void dgemm_mfma_kernel(int M, int N, int K, double alpha,
-                                                         const double* __restrict__ A, int64_t lda,
-                                                         const double* __restrict__ B, int64_t ldb,
-                                                         double beta, double* __restrict__ C,
-                                                         int64_t ldc) {
-  constexpr int TM = 128, TN = 128, TK = 16;
-  __shared__ double As[TK][TM + 2];
-  __shared__ double Bs[TK][TN + 2];
-  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = lane >> 4, c = lane & 15;
-  const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  f64x4 acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0.0;
-  for (int k0 = 0; k0 < K; k0 += TK) {
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const int e = tid + 256 * l;
-      int ii, kk;
-      if (TA) { ii = e & (TM - 1); kk = e >> 7; } else { kk = e & (TK - 1); ii = e >> 4; }
-      const int gi = i0 + ii, gk = k0 + kk;
-      double v = 0.0;
-      if (gi < M && gk < K) v = TA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
-      As[kk][ii] = v;
-    }
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const int e = tid + 256 * l;
-      int jj, kk;
-      if (TB) { kk = e & (TK - 1); jj = e >> 4; } else { jj = e & (TN - 1); kk = e >> 7; }
-      const int gj = j0 + jj, gk = k0 + kk;
-      double v = 0.0;
-      if (gj < N && gk < K) v = TB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
-      Bs[kk][jj] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < TK; kk += 4) {
-      double a[4], b[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk + g][wi + r * 16 + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) b[r] = Bs[kk + g][wj + r * 16 + c];
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  // v_mfma_f64_16x16x4f64 accumulator layout: lane (g, c), element r ->
-  // row g + 4r, column c
-#pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = i0 + wi + rb * 16 + g + 4 * r;
-        const int gj = j0 + wj + cb * 16 + c;
-        if (gi >= M || gj >= N) continue;
-        double* p = C + (int64_t)gi * ldc + gj;
-        *p = (beta == 0.0) ? alpha * acc[rb][cb][r] : alpha * acc[rb][cb][r] + beta * (*p);
-      }
-}
-
-template <bool TA, bool TB>
-static int dgemm(hipStream_t st, int M, int N, int K, double alpha, const double* A, int64_t lda,
-                 const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
-  if (M <= 0 || N <= 0) return 0;
-  dim3 grid((N + 127) / 128, (M + 127) / 128);
-  hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB>), grid, dim3(256), 0, st, M, N, K, alpha, A, lda, B,
-                     ldb, beta, C, ldc);
+  constexpr int V = 16 / (int)sizeof(T);
+  const bool vec = (lda % V == 0) && (ldb % V == 0) && (K % V == 0) &&
+                   (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+  dim3 grid((unsigned)((N + NT_TN - 1) / NT_TN), (unsigned)((M + NT_TM - 1) / NT_TM));
+  if (vec)
+    hipLaunchKernelGGL((gemm_nt_kernel<T, true>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, alpha,
+                       A, lda, B, ldb, beta, C, ldc, lower_only);
+  else
+    hipLaunchKernelGGL((gemm_nt_kernel<T, false>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, alpha,
+                       A, lda, B, ldb, beta, C, ldc, lower_only);
   SMX_LAUNCHED();
   return 0;
 }
@@ -266,7 +269,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int AG_KR = 128;   // rows of R per workgroup
 constexpr int AG_BN = 256;   // columns of M per workgroup
-constexpr int AG_KC = 64;    // reduction chunk
+constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
 
 __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
   // 2^(target - ceil(log2 amax)), or 1 for an all-zero row
@@ -276,7 +279,7 @@ __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
   return ldexpf(1.f, target_exp - e);
 }
 
-// M (n, n) FP32 -> scaled fp16 hi / lo (npad, npad), zero padded.
+// M (n, n) FP64 -> scaled fp16 hi / lo (npad, npad), zero padded.
 __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t npad,
                                const float* __restrict__ rho_p, _Float16* __restrict__ Mh,
                                _Float16* __restrict__ Ml) {
@@ -290,23 +293,83 @@ __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t 
   Ml[idx] = static_cast<_Float16>(v - static_cast<double>(h));
 }
 
+// R (FP32, Kp x npad) -> scaled fp16 hi / lo with each row's power-of-two
+// scale (max |R_k| -> 2^14).
+__global__ void split_r_kernel(const float* __restrict__ R, int64_t total, int64_t npad,
+                               const unsigned* __restrict__ rowmax, _Float16* __restrict__ Rh,
+                               _Float16* __restrict__ Rl) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int64_t k = idx / npad;
+  const float v = R[idx] * pow2_scale_for(__uint_as_float(rowmax[k]), 14);
+  const _Float16 h = static_cast<_Float16>(v);
+  Rh[idx] = h;
+  Rl[idx] = static_cast<_Float16>(v - static_cast<float>(h));
+}
+
+// One chunk (KC reduction indices) of the ADMM product's operands.  B: 256
+// rows of M (hi and lo), thread t moves 16-byte segment t%TPR of row
+// t/TPR + (256/TPR)p, TPR = KC/8 lanes covering one row piece; A: 128 rows of
+// R (hi and lo), same pattern.  Loading the next chunk into registers during
+// the MFMAs (a 64-wide chunk at one workgroup per CU, or a 32-wide one) spills
+// and runs at half the speed of this load-after-compute form at two
+// workgroups per CU.
+template <int KC>
+struct AdmmStage {
+  static constexpr int TPR = KC / 8;
+  static constexpr int RPP = 256 / TPR;
+  static constexpr int PB_ = AG_BN / RPP;   // passes for B
+  static constexpr int PA_ = AG_KR / RPP;   // passes for A
+  uint4 bh[PB_], bl[PB_];
+  __device__ void load_b(const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml,
+                         int64_t npad, int64_t j0, int64_t i0) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PB_; ++p) {
+      const int64_t off = (j0 + t / TPR + RPP * p) * npad + i0 + 8 * (t % TPR);
+      bh[p] = *reinterpret_cast<const uint4*>(Mh + off);
+      bl[p] = *reinterpret_cast<const uint4*>(Ml + off);
+    }
+  }
+  template <int P>
+  __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P], _Float16 (*Al)[P],
+                        const _Float16* __restrict__ Rh, const _Float16* __restrict__ Rl,
+                        int64_t npad, int64_t k0, int64_t i0) const {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PB_; ++p) {
+      *reinterpret_cast<uint4*>(&Bh[t / TPR + RPP * p][8 * (t % TPR)]) = bh[p];
+      *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
+    }
+#pragma unroll
+    for (int p = 0; p < PA_; ++p) {
+      const int64_t off = (k0 + t / TPR + RPP * p) * npad + i0 + 8 * (t % TPR);
+      *reinterpret_cast<uint4*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) =
+          *reinterpret_cast<const uint4*>(Rh + off);
+      *reinterpret_cast<uint4*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) =
+          *reinterpret_cast<const uint4*>(Rl + off);
+    }
+  }
+};
+
+template <int KC>
 __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
-    int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
-    const unsigned* __restrict__ rowmax, const _Float16* __restrict__ Mh,
+    int64_t Kp, int64_t npad, int chunks_per_split, const _Float16* __restrict__ Rh,
+    const _Float16* __restrict__ Rl, const _Float16* __restrict__ Mh,
     const _Float16* __restrict__ Ml, float* __restrict__ part) {
-  __shared__ __align__(16) _Float16 Ah[AG_KR][AG_KC + 8];
-  __shared__ __align__(16) _Float16 Al[AG_KR][AG_KC + 8];
+  constexpr int P = KC + 8;
+  __shared__ __align__(16) _Float16 Bh[AG_BN][P];
+  __shared__ __align__(16) _Float16 Bl[AG_BN][P];
+  __shared__ __align__(16) _Float16 Ah[AG_KR][P];
+  __shared__ __align__(16) _Float16 Al[AG_KR][P];
   const int64_t j0 = (int64_t)blockIdx.x * AG_BN;
   const int split = blockIdx.y;
   const int64_t k0 = (int64_t)blockIdx.z * AG_KR;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 5, c = lane & 31;
-  const int64_t nch = npad / AG_KC;
+  const int64_t nch = npad / KC;
   const int64_t ch0 = (int64_t)split * chunks_per_split;
   const int64_t ch1 = ch0 + chunks_per_split < nch ? ch0 + chunks_per_split : nch;
-  // staging role: row sr, 32 columns starting at sc0
-  const int sr = tid >> 1, sc0 = (tid & 1) * 32;
-  const float rs = pow2_scale_for(__uint_as_float(rowmax[k0 + sr]), 14);
   f32x16 acc[4][2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -314,50 +377,38 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  const int64_t jrow0 = j0 + wave * 64 + c;          // M row read by this lane (cb = 0)
+  AdmmStage<KC> stg;
+  if (ch0 < ch1) {
+    stg.load_b(Mh, Ml, npad, j0, ch0 * KC);
+    stg.store(Bh, Bl, Ah, Al, Rh, Rl, npad, k0, ch0 * KC);
+  }
+  __syncthreads();
   for (int64_t ch = ch0; ch < ch1; ++ch) {
-    const int64_t i0 = ch * AG_KC;
-    {
-      const float4* src = reinterpret_cast<const float4*>(R + (k0 + sr) * npad + i0 + sc0);
+    const bool more = ch + 1 < ch1;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float4 v = src[q];
-        const float e[4] = {v.x * rs, v.y * rs, v.z * rs, v.w * rs};
+    for (int t = 0; t < KC / 16; ++t) {
+      half8 bh[2], bl[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const _Float16 h = static_cast<_Float16>(e[u]);
-          Ah[sr][sc0 + 4 * q + u] = h;
-          Al[sr][sc0 + 4 * q + u] = static_cast<_Float16>(e[u] - static_cast<float>(h));
-        }
+      for (int cb = 0; cb < 2; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
+        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
       }
-    }
-    // B fragments: 32 consecutive halves of row (jrow0 + 32 cb) at i0 + 32 g
-    half8 bh[2][4], bl[2][4];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int64_t off = (jrow0 + 32 * cb) * npad + i0 + 32 * g;
-      const half8* ph = reinterpret_cast<const half8*>(Mh + off);
-      const half8* pl = reinterpret_cast<const half8*>(Ml + off);
+      for (int rb = 0; rb < 4; ++rb) {
+        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
+        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        bh[cb][t] = ph[t];
-        bl[cb][t] = pl[t];
+        for (int cb = 0; cb < 2; ++cb) {
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb) {
-        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][32 * g + 8 * t]);
-        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][32 * g + 8 * t]);
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb][t], acc[rb][cb], 0, 0, 0);
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb][t], acc[rb][cb], 0, 0, 0);
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb][t], acc[rb][cb], 0, 0, 0);
-        }
-      }
+    if (more) {
+      stg.load_b(Mh, Ml, npad, j0, (ch + 1) * KC);
+      stg.store(Bh, Bl, Ah, Al, Rh, Rl, npad, k0, (ch + 1) * KC);
     }
     __syncthreads();
   }
@@ -499,7 +550,7 @@ struct NnlsWs {
   float* Ut;      // n x ldu
   float* Q;       // n x n FP32 Gram
   double* Q64;    // n x n FP64, inverted in place
-  double* work64; // n*n/2 work of the recursive inverse
+  double* work64; // n*n + n work of the recursive inverse
   float* Yf;      // K x ldu
   float* C;       // K x n
   double* z;      // K x n
@@ -512,6 +563,8 @@ struct NnlsWs {
   _Float16* Mh;     // npad x npad
   _Float16* Ml;     // npad x npad
   float* Rp;        // Kp x npad
+  _Float16* Rh;     // Kp x npad
+  _Float16* Rl;     // Kp x npad
   float* part;      // S x Kp x npad
   unsigned* rmax;   // 3 x Kp
   int64_t npad, Kp;
@@ -519,13 +572,14 @@ struct NnlsWs {
   int64_t bytes;
 };
 
-// splits of the reduction so that the ADMM product fills two workgroups per CU
+// splits of the reduction so that the ADMM product fills the chip once
+// (two workgroups on each of the 256 CUs)
 static int admm_splits(int64_t npad, int64_t Kp) {
   const int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);
   int64_t S = 512 / (tiles > 0 ? tiles : 1);
   if (S < 1) S = 1;
   if (S > 16) S = 16;
-  const int64_t nch = npad / AG_KC;
+  const int64_t nch = npad / AG_KP;
   if (S > nch) S = nch;
   return static_cast<int>(S);
 }
@@ -543,7 +597,7 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Ut = reinterpret_cast<float*>(take(n * ldu * 4));
   w.Q = reinterpret_cast<float*>(take(n * n * 4));
   w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
-  w.work64 = reinterpret_cast<double*>(take((n * n / 2 + n) * 8));
+  w.work64 = reinterpret_cast<double*>(take((n * n + n) * 8));
   w.Yf = reinterpret_cast<float*>(take(K * ldu * 4));
   w.C = reinterpret_cast<float*>(take(K * n * 4));
   w.z = reinterpret_cast<double*>(take(K * n * 8));
@@ -559,6 +613,8 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Mh = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Ml = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Rp = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
+  w.Rh = reinterpret_cast<_Float16*>(take(w.Kp * w.npad * 2));
+  w.Rl = reinterpret_cast<_Float16*>(take(w.Kp * w.npad * 2));
   w.part = reinterpret_cast<float*>(take((int64_t)w.S * w.Kp * w.npad * 4));
   w.rmax = reinterpret_cast<unsigned*>(take(3 * w.Kp * 4));
   w.bytes = off;
@@ -642,6 +698,9 @@ __global__ void mirror_lower_kernel(T* __restrict__ A, int64_t n, int64_t lda) {
   if (j > i) A[i * lda + j] = A[j * lda + i];
 }
 
+// Every product is in NT form (both operands k-contiguous):
+//   Tt = X A21^T,  T = Tt^T,  A22 -= T A21^T,  A21 = -S^-1 Tt^T,  A12 = A21^T,
+//   A11 -= Tt A12^T.  work holds 2 n1 n2 per level, n^2 in total suffices.
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
     hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
@@ -651,24 +710,24 @@ static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double
   const int64_t n1 = ((n / 2 + INV_BASE - 1) / INV_BASE) * INV_BASE;
   const int64_t n2 = n - n1;
   double* A11 = A;
+  double* A12 = A + n1;
   double* A21 = A + n1 * lda;
   double* A22 = A + n1 * lda + n1;
-  double* Tm = work;                      // n2 x n1
-  double* deeper = work + n2 * n1;
+  double* Tt = work;                      // n1 x n2
+  double* Tm = work + n1 * n2;            // n2 x n1
+  double* deeper = work + 2 * n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
-  if (int e = dgemm<false, false>(st, (int)n2, (int)n1, (int)n1, 1.0, A21, lda, A11, lda, 0.0, Tm, n1))
-    return e;
-  if (int e = dgemm<false, true>(st, (int)n2, (int)n2, (int)n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda))
-    return e;
-  if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
-  if (int e = dgemm<false, false>(st, (int)n2, (int)n1, (int)n2, -1.0, A22, lda, Tm, n1, 0.0, A21, lda))
-    return e;
-  if (int e = dgemm<true, false>(st, (int)n1, (int)n1, (int)n2, -1.0, Tm, n1, A21, lda, 1.0, A11, lda))
-    return e;
+  if (int e = gemm_nt<double>(st, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, Tt, n2)) return e;
   hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
-                     A21, lda, n2, n1, A + n1, lda);
+                     Tt, n2, n1, n2, Tm, n1);
   SMX_LAUNCHED();
-  return 0;
+  if (int e = gemm_nt<double>(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda)) return e;
+  if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
+  if (int e = gemm_nt<double>(st, n2, n1, n2, -1.0, A22, lda, Tt, n2, 0.0, A21, lda)) return e;
+  hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
+                     A21, lda, n2, n1, A12, lda);
+  SMX_LAUNCHED();
+  return gemm_nt<double>(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda);
 }
 
 // Q (FP32, full) -> FP64 with Q += reg I, rho = tr(Q)/n, Q += rho I (nnls.py:115-121)
@@ -728,8 +787,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
                      n, m, ldu, row_scale, w.Ut);
   SMX_LAUNCHED();
   // Q = U^T U: lower half on MFMA (FP32), mirrored, widened to FP64
-  if (int e = gemm<false, true>(st, (int)n, (int)n, (int)ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1))
-    return e;
+  if (int e = gemm_nt<float>(st, n, n, ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1)) return e;
   hipLaunchKernelGGL(mirror_lower_kernel<float>, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n, n);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
@@ -747,8 +805,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(to_float_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m, ldu,
                      w.Yf);
   SMX_LAUNCHED();
-  if (int e = gemm<false, true>(st, (int)K, (int)n, (int)ldu, -1.f, w.Yf, ldu, w.Ut, ldu, 0.f, w.C, n))
-    return e;
+  if (int e = gemm_nt<float>(st, K, n, ldu, -1.f, w.Yf, ldu, w.Ut, ldu, 0.f, w.C, n)) return e;
   const int64_t tot = K * n;
   const int64_t npad = w.npad, Kp = w.Kp;
   const int S = w.S;
@@ -769,7 +826,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   const int check = 25;
   std::vector<unsigned> dl(K), zm(K);
   int done = 0;
-  const int64_t nch = npad / AG_KC;
+  const int64_t nch = npad / AG_KP;
   const int cps = static_cast<int>((nch + S - 1) / S);
   dim3 ggrid((unsigned)(npad / AG_BN), (unsigned)S, (unsigned)(Kp / AG_KR));
   stage_begin(ST_ADMM, st);
@@ -780,9 +837,13 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     unsigned* rcur = w.rmax + (it % 3) * Kp;
     unsigned* rnext = w.rmax + ((it + 1) % 3) * Kp;
     unsigned* rclear = w.rmax + ((it + 2) % 3) * Kp;
-    hipLaunchKernelGGL(admm_gemm_kernel, ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rp, rcur, w.Mh,
-                       w.Ml, w.part);
+    hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
+                       Kp * npad, npad, rcur, w.Rh, w.Rl);
+    stage_begin(ST_ADMM_PRODUCT, st);
+    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rh,
+                       w.Rl, w.Mh, w.Ml, w.part);
     SMX_LAUNCHED();
+    stage_end(ST_ADMM_PRODUCT, st, 1);
     hipLaunchKernelGGL(admm_update_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
                        K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext, rclear,
                        1.6, w.active, w.delta);
@@ -813,30 +874,24 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   return 0;
 }
 
-int smx_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, float alpha,
-                 const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
-                 int64_t ldc, void* stream) {
-  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0, "smx_gemm_f32: bad arguments");
-  hipStream_t st = as_stream(stream);
-  SMX_REQUIRE(!(trans_a && trans_b), "smx_gemm_f32: both transposed is not supported");
-  if (trans_a) return gemm<true, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (trans_b) return gemm<false, true>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
-  return gemm<false, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+int smx_gemm_nt_f32(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
+                    const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream) {
+  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0 && M < (1LL << 31) && N < (1LL << 31) &&
+                  K < (1LL << 31),
+              "smx_gemm_nt_f32: bad arguments");
+  return gemm_nt<float>(as_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
-int smx_gemm_f64(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K, double alpha,
-                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
-                 int64_t ldc, void* stream) {
-  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0, "smx_gemm_f64: bad arguments");
-  SMX_REQUIRE(!(trans_a && trans_b), "smx_gemm_f64: both transposed is not supported");
-  hipStream_t st = as_stream(stream);
-  if (trans_a) return dgemm<true, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (trans_b) return dgemm<false, true>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
-  return dgemm<false, false>(st, (int)M, (int)N, (int)K, alpha, A, lda, B, ldb, beta, C, ldc);
+int smx_gemm_nt_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                    const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream) {
+  SMX_REQUIRE(A && B && C && M >= 0 && N >= 0 && K >= 0 && M < (1LL << 31) && N < (1LL << 31) &&
+                  K < (1LL << 31),
+              "smx_gemm_nt_f64: bad arguments");
+  return gemm_nt<double>(as_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream) {
-  SMX_REQUIRE(A && work && n > 0 && n < ((int64_t)1 << 31), "smx_spd_inverse_f64: bad arguments");
+  SMX_REQUIRE(A && work && n > 0 && n < ((int64_t)1 << 31), "smx_spd_inverse_f64: bad arguments");  // work: n*n + n doubles
   return inv_spd_rec(as_stream(stream), A, n, n, work);
 }
 

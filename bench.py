@@ -48,10 +48,7 @@ def make_workload(config, seed):
     A_xy = lib[:, P:P + B].T
     A_h = [lib[:, P + B * k:P + B * (k + 1)].T for k in (1, 2, 3, 4)]
     y_R = M.density_target(pb, dd).cpu().numpy()
-    synth.observe(dd, A_xy, A_h, y_R.sum() / lib.shape[0], pb['mass_to_light_ratio'])
-    floor = 1e-3 * np.median(dd['XY_density_data'][dd['XY_density_data'] > 0])
-    dd['XY_density_data'] = np.maximum(dd['XY_density_data'], floor)
-    dd['XY_density_data_err'] = 0.01 * dd['XY_density_data'] + 1e-12
+    synth.observe(dd, A_xy, A_h, y_R.sum() / lib.shape[0], pb['mass_to_light_ratio'], floor=1e-3)
     valid = float(libd['valid'].mean().item())
     del libd, wb, Tt, dt0
     torch.cuda.empty_cache()
@@ -168,9 +165,20 @@ def main():
     T = 1000 if args.config != 'plain-2k' else 500
     n = dd['w0'].shape[0]
 
-    def step():
+    grad_mode = args.config.startswith('grad')
+    if grad_mode:
+        from schwarmax_amd import grad as GR
+
+    def forward():
         return L.model_likelihood(theta, dd, B, P, schedule=sched, nnls_maxiter=T,
                                   nnls_tol=args.nnls_tol)
+
+    def step():
+        if grad_mode:
+            v, g = GR.model_likelihood_and_grad(theta, dd, B, P, schedule=sched, nnls_maxiter=T,
+                                                nnls_tol=args.nnls_tol, ckpt_every=64)
+            return v
+        return forward()
 
     for i in range(args.warmup):
         t0 = time.time()
@@ -188,6 +196,14 @@ def main():
     stages = _lib.timing_read()
     _lib.timing_enable(False)
     log('logL', [float(v) for v in vals], f'{dt / args.steps:.3f} s/eval')
+    fwd_s = None
+    if grad_mode:
+        torch.cuda.synchronize()
+        t1 = time.time()
+        forward()
+        torch.cuda.synchronize()
+        fwd_s = time.time() - t1
+        log(f'forward alone {fwd_s:.3f} s; gradient / forward = {dt / args.steps / fwd_s:.2f}')
     log('stages (ms per eval):', {k: round(v[0] / args.steps, 2) for k, v in stages.items()})
 
     # Roofline of the dominant kernel: whichever of the ADMM product (HBM
@@ -214,7 +230,8 @@ def main():
                 'kernel': 'admm_gemm_kernel', 'per_launch': prod_bytes,
                 'ms_per_launch': prod_ms, 'launches': stages['admm_product'][1]}
     out = {
-        'metric': 'model evals/s @20k orbits (plain)', 'value': args.steps / dt,
+        'metric': ('gradient evals/s @20k orbits' if grad_mode else
+                   'model evals/s @20k orbits (plain)'), 'value': args.steps / dt,
         'unit': 'model evals/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': 1e3 * dt / args.steps, 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic',
@@ -224,6 +241,9 @@ def main():
         'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
         'roofline': roof,
     }
+    if grad_mode:
+        out['gradient_to_forward'] = (dt / args.steps) / fwd_s
+        out['config']['ckpt_every'] = 64
     if not args.no_cpu:
         log('cpu baseline ...')
         out['cpu_baseline'] = cpu_baseline(args.config, args.seed)

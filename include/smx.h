@@ -137,6 +137,53 @@ int smx_density_target(const smx_potential* p, const double* R_grid, const doubl
                        const double* phi_grid, int64_t P, double dR, double dz, double dphi,
                        const double* sample, int64_t S, double* out, void* stream);
 
+/* ---- derivatives (jax.grad of likelihood.py:96-244 through model.py:70-258) */
+/* The potential's theta-dependent fields, in this order, are the coordinates
+ * of every field cotangent / Jacobian below: nfw_mGM, nfw_Rs, mn_mGM,
+ * mn_rho0, mn_Rs, mn_Hs, t3_M, t3_L, t3_a, t3_b, v4_M.  The host maps theta
+ * to the fields (likelihood.py:105-165) and differentiates that map itself. */
+#define SMX_POT_FIELDS 11
+
+/* smx_build_library that also stores, for the adjoint, the state
+ * ckpt[(j, i, 0:8)] = (y[6], dt, t) of integration i before attempt
+ * j * ckpt_every, j < ceil(N_max / ckpt_every).  The slabs in `workspace`
+ * stay valid until the caller reuses it; smx_library_vjp reads them. */
+int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                           const double* wb, const double* T_total, const double* dt0,
+                           int64_t n, int32_t r, const int32_t* bin_mapping,
+                           const double* num_per_bin, const double* v0, const double* sdisp,
+                           void* workspace, double* lib, double* valid, double* T_int,
+                           double* t_final, double* ckpt, int32_t ckpt_every, void* stream);
+/* Scratch for smx_library_vjp: per integration ckpt_every * 31 doubles of
+ * replayed attempts plus two scalars. */
+int64_t smx_library_vjp_workspace_bytes(int64_t n_int, int32_t ckpt_every);
+/* Vector-Jacobian product of the library (integrates.py:37-300 with the
+ * realization average; bin membership, accept decisions and validity fixed,
+ * as jax.grad treats them).  lib_bar (n, P + 6B) is the cotangent of lib.
+ * Outputs per integration i < n*r: wb_bar (i, 6), dt0_bar (i), field_bar
+ * (i, SMX_POT_FIELDS), omega_bar (i) (the ODE's Omega = -Omega_bar) and
+ * rot_bar (i, 3) (the sky rotation's row 2, which sets v_los); the caller
+ * sums over integrations.  fwd_workspace, valid, T_int and ckpt are those of
+ * the smx_build_library_ckpt call that made lib. */
+int smx_library_vjp(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                    const double* wb, const double* dt0, int64_t n, int32_t r,
+                    const int32_t* bin_mapping, const double* num_per_bin, const double* v0,
+                    const double* sdisp, const void* fwd_workspace, const double* valid,
+                    const double* T_int, const double* ckpt, int32_t ckpt_every,
+                    const double* lib_bar, void* workspace, double* wb_bar, double* dt0_bar,
+                    double* field_bar, double* omega_bar, double* rot_bar, void* stream);
+/* Jacobians of smx_initial_conditions with respect to the fields
+ * (jeans.py:9-94, model.py:117-169): wb_jac (n, r, 6, SMX_POT_FIELDS),
+ * dt0_jac (n, r, SMX_POT_FIELDS).  T_total only sets validity and has none. */
+int smx_initial_conditions_jac(const smx_potential* p, const double* w0, const double* normals,
+                               const double* uniforms, int64_t n, int32_t r, int32_t r_total,
+                               double N_step, double* wb_jac, double* dt0_jac, void* stream);
+/* Jacobian of smx_density_target with respect to the fields (model.py:192-213):
+ * jac (P, SMX_POT_FIELDS). */
+int smx_density_target_jac(const smx_potential* p, const double* R_grid, const double* z_grid,
+                           const double* phi_grid, int64_t P, double dR, double dz, double dphi,
+                           const double* sample, int64_t S, double* jac, void* stream);
+
 /* ---- NNLS (nnls.py:6-166) ----------------------------------------------- */
 /* Bytes of NNLS workspace for n orbits, m rows, K right-hand sides. */
 int64_t smx_nnls_workspace_bytes(int64_t n, int64_t m, int64_t K);

@@ -56,24 +56,50 @@ def _clip(a, lo, hi):
 
 def rk_step(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max):
     """One BS3 attempt (integrates.py:90-112).  Returns (t, y, dt, k1, dt_out)."""
+    return rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max)[:5]
+
+
+def rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max, dec_in=None):
+    """rk_step that also returns its branch decisions; ``dec_in`` replays them.
+
+    The decisions are where the attempt's map is not smooth: accept / reject,
+    which of |y| and |y_new| the error scale takes, whether the error floor
+    1e-10 and the two clips of the controller are active.  jax.grad
+    differentiates the branch taken; replaying the branches of a base run
+    keeps nearby evaluations on that branch, for finite differences.
+    Returns (t, y, dt, k1, dt_out, decisions).
+    """
     e1, e2, e3, e4 = -5.0 / 72.0, 1.0 / 12.0, 1.0 / 9.0, -1.0 / 8.0
     k2 = deriv_barred(y + 0.5 * dt * k1, acc_fn, Omega)
     k3 = deriv_barred(y + 0.75 * dt * k2, acc_fn, Omega)
     y_new = y + dt * (2.0 / 9.0 * k1 + 1.0 / 3.0 * k2 + 4.0 / 9.0 * k3)
     k4 = deriv_barred(y_new, acc_fn, Omega)
     err_vec = dt * (e1 * k1 + e2 * k2 + e3 * k3 + e4 * k4)
-    scale = atol + rtol * np.maximum(np.fabs(y), np.fabs(y_new))
+    ay, an = np.fabs(y), np.fabs(y_new)
+    sel = ay >= an if dec_in is None else dec_in['sel']
+    scale = atol + rtol * np.where(sel, ay, an)
     es = (err_vec / scale) ** 2
     err_norm = np.sqrt((((((es[0] + es[1]) + es[2]) + es[3]) + es[4]) + es[5]) / 6.0)
-    accept = err_norm <= 1.0
+    accept = err_norm <= 1.0 if dec_in is None else dec_in['accept']
     y_out = np.where(accept, y_new, y)
     t_new = np.where(accept, t + dt, t)
     k1_new = np.where(accept, k4, k1)
-    sf = 0.9 * np.power(_nan_max(err_norm, 1e-10), -1.0 / 3.0)
-    sf = _clip(sf, 0.2, 5.0)
-    dt_new = _clip(dt * sf, dt_min, dt_max)
+    floor = (err_norm < 1e-10) if dec_in is None else dec_in['floor']
+    sf_raw = 0.9 * np.power(np.where(floor, 1e-10, err_norm), -1.0 / 3.0)
+    if dec_in is None:
+        sf_st = np.where(sf_raw < 0.2, -1, np.where(sf_raw > 5.0, 1, 0))
+    else:
+        sf_st = dec_in['sf']
+    sf = np.where(sf_st < 0, 0.2, np.where(sf_st > 0, 5.0, sf_raw))
+    dtp = dt * sf
+    if dec_in is None:
+        dt_st = np.where(dtp < dt_min, -1, np.where(dtp > dt_max, 1, 0))
+    else:
+        dt_st = dec_in['dt']
+    dt_new = np.where(dt_st < 0, dt_min, np.where(dt_st > 0, dt_max, dtp))
     dt_out = np.where(accept, dt, 0.0)
-    return t_new, y_out, dt_new, k1_new, dt_out
+    dec = {'accept': accept, 'sel': sel, 'floor': floor, 'sf': sf_st, 'dt': dt_st}
+    return t_new, y_out, dt_new, k1_new, dt_out, dec
 
 
 # --------------------------------------------------------------------------
@@ -171,13 +197,25 @@ def _segsum(M, nseg, owner, idx, w):
 
 
 def integrate_chunked(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, rtol,
-                      dt_min, dt_max, grids, chunk_size, symmetry=True, want_h0=True):
+                      dt_min, dt_max, grids, chunk_size, symmetry=True, want_h0=True, tape=None):
     """integrate_adaptive_barred_chunked (integrates.py:37-238) for M integrations.
 
     w0: (M, 6); T_total, dt_init: (M,).  Returns dict of per-integration
     Rzphi (M,P), surface_density (M,B), h0..h4 (M,B), valid (M,), T (M,),
     t_final (M,).
+
+    ``tape``: None, or a dict with 'mode' 'record' (the discrete decisions of
+    this run -- accept flags, bin ids, validity -- are stored in it) or
+    'replay' (the stored decisions are used instead of recomputed), so that
+    nearby parameters are evaluated on the same piece of the piecewise-smooth
+    map jax.grad differentiates.
     """
+    rec = tape is not None and tape['mode'] == 'record'
+    rep = tape is not None and tape['mode'] == 'replay'
+    if rec:
+        tape.update(accept=[], irz=[], vbin=[])
+    it_acc = iter(tape['accept']) if rep else None
+    it_bin = iter(zip(tape['irz'], tape['vbin'])) if rep else None
     w0 = np.asarray(w0, dtype=np.float64)
     M = w0.shape[0]
     B = grids.num_Vbin
@@ -197,7 +235,11 @@ def integrate_chunked(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, 
     for _ in range(N_max // chunk_size):
         ys, dts = [], []
         for _ in range(chunk_size):
-            t, y, dt, k1, dt_out = rk_step(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max)
+            t, y, dt, k1, dt_out, dec = rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol,
+                                                    dt_min, dt_max,
+                                                    next(it_acc) if rep else None)
+            if rec:
+                tape['accept'].append(dec)
             ys.append(y)
             dts.append(dt_out)
         yc = np.stack(ys, axis=1)                         # (6, C, M)
@@ -205,6 +247,11 @@ def integrate_chunked(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, 
         ysym = (signs[:, :, None, None] * yc[None]).transpose(1, 0, 2, 3).reshape(6, -1)
         dsym = np.broadcast_to(dc, (nsym, chunk_size, M)).reshape(-1)
         irz, vbin, vlos = grids.indices(ysym)
+        if rec:
+            tape['irz'].append(irz)
+            tape['vbin'].append(vbin)
+        if rep:
+            irz, vbin = next(it_bin)
         rz += _segsum(M, grids.P, owner, irz, dsym)
         counts += _segsum(M, B, owner, vbin, dsym)
         g, H = grids.gh_terms(vbin, vlos)
@@ -215,6 +262,10 @@ def integrate_chunked(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, 
         T_acc += dsym.reshape(nsym * chunk_size, M).sum(axis=0)
     EJ = compute_EJ(y, pot_fn, Omega)
     valid = np.where((np.fabs(EJ / EJ0 - 1.0) < 0.1) & (t > np.asarray(T_total) / 10), 1.0, 0.0)
+    if rec:
+        tape['valid'] = valid
+    if rep:
+        valid = tape['valid']
     T_int = T_acc + EPSILON
     v = valid[:, None] > 0.5
     out = {

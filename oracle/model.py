@@ -99,7 +99,7 @@ def compute_model_single(w, A_Rzphi, A_xy, A_h, gamma_kin, v0, s):
 
 
 def _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_lim_grid,
-             xy_lim_grid, xy_n_grid, sched, traj):
+             xy_lim_grid, xy_n_grid, sched, traj, tape=None):
     s = sched
     r = 1 if traj else s['n_realizations']
     wb, Tt, dt0 = initial_conditions(dict_data, pb, ph, r, s['N_step_per_orb'], s['N_dynamical_time'])
@@ -113,7 +113,7 @@ def _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_li
     if traj:
         per = I.integrate_withtraj(*args)
     else:
-        per = I.integrate_chunked(*args, chunk_size=s['chunk'])
+        per = I.integrate_chunked(*args, chunk_size=s['chunk'], tape=tape)
     lib = I.realization_average(per, n, r)
     lib['per'] = per
     return lib
@@ -121,7 +121,7 @@ def _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_li
 
 def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
              Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid,
-             nnls_maxiter, regularization, schedule, traj, channel_w):
+             nnls_maxiter, regularization, schedule, traj, channel_w, tape=None, W_fixed=None):
     if density_func is not P.density_func or potential_func is not P.potential_func:
         raise NotImplementedError("only the canonical density/potential are restated")
     sched = schedule_of(schedule)
@@ -131,7 +131,7 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     Omega_bar = params_disk_rho['Omega_bar']
     rot = make_rotation_matrix(params_disk_rho['alpha'], params_disk_rho['beta'], params_disk_rho['gamma'])
     lib = _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_lim_grid,
-                   xy_lim_grid, xy_n_grid, sched, traj)
+                   xy_lim_grid, xy_n_grid, sched, traj, tape)
     A_Rzphi = lib['Rzphi'].T
     A_xy = lib['surface_density'].T
     A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
@@ -164,10 +164,13 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     y_xy_boot = y_xy[None, :] + dd['XY_standard_normal'] * sig_xy[None, :]
     y_h_boot = [y_h[k][None, :] + dd['h%d_standard_normal' % (k + 1)] * sig_h[k][None, :]
                 for k in range(4)]
-    W = NN.solve_nnls_admm_bootstrap(A_Rzphi, A_xy, A_h, y_Rzphi, y_xy, gamma_kin,
-                                     y_xy_boot, y_h_boot, sig_Rzphi, sig_xy, sig_h,
-                                     lambda_reg=regularization, maxiter=nnls_maxiter,
-                                     w_rzphi=channel_w[0], w_xy=channel_w[1], w_h=channel_w[2])
+    if W_fixed is None:
+        W = NN.solve_nnls_admm_bootstrap(A_Rzphi, A_xy, A_h, y_Rzphi, y_xy, gamma_kin,
+                                         y_xy_boot, y_h_boot, sig_Rzphi, sig_xy, sig_h,
+                                         lambda_reg=regularization, maxiter=nnls_maxiter,
+                                         w_rzphi=channel_w[0], w_xy=channel_w[1], w_h=channel_w[2])
+    else:
+        W = np.asarray(W_fixed, dtype=np.float64)   # stop_gradient'ed weights (nnls.py:165)
     d3, d2, h1, h2, h3, h4, V, sig = compute_model_single(W, A_Rzphi, A_xy, A_h, gamma_kin, v0_ap, s_ap)
     d2 = d2 * mmpo * lm
     out = {'density_3d': y_Rzphi, 'density_3d_model': d3, 'surface_density': d2,
@@ -180,12 +183,14 @@ def build_model(density_func, potential_func, params_halo_pot, params_disk_rho, 
                 Rzphi_n_tot=360, Rzphi_n_grid=(10, 6, 6),
                 Rzphi_lim_grid=((0, 10.), (-3, 3), (-np.pi, np.pi)),
                 xy_lim_grid=((-10., 10.), (-3., 3.)), xy_n_grid=(60, 40),
-                nnls_maxiter=200, regularization=1.0, schedule=None, return_library=False):
-    """model.py:70-286."""
+                nnls_maxiter=200, regularization=1.0, schedule=None, return_library=False,
+                tape=None, W_fixed=None):
+    """model.py:70-286.  ``tape`` / ``W_fixed``: see integrate_chunked and
+    model_likelihood_frozen."""
     out, lib, _ = _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data,
                            num_Vbin, Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid,
                            xy_n_grid, nnls_maxiter, regularization, schedule, False,
-                           (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0)))
+                           (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0)), tape, W_fixed)
     if return_library:
         return out, lib
     return out
@@ -294,11 +299,44 @@ def bootstrap_loglike(model_dict, dict_data, sigma_amplify=1.0):
 
 
 def model_likelihood(params, dict_data, num_Vbin, Rzphi_n_tot, sigma_amplify=1.0,
-                     schedule=None, nnls_maxiter=200):
+                     schedule=None, nnls_maxiter=200, tape=None, W_fixed=None, return_weights=False):
     """likelihood.py:96-244."""
     ph, pb = theta_to_params(params)
     Rzphi_lim, Rzphi_n, xy_lim, xy_n = grid_args(dict_data)
     md = build_model(P.density_func, P.potential_func, ph, pb, dict_data, num_Vbin, Rzphi_n_tot,
                      Rzphi_n, Rzphi_lim, xy_lim, xy_n, nnls_maxiter=nnls_maxiter,
-                     regularization=1.0, schedule=schedule)
-    return bootstrap_loglike(md, dict_data, sigma_amplify)[0]
+                     regularization=1.0, schedule=schedule, tape=tape, W_fixed=W_fixed)
+    val = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
+    return (val, md['weights']) if return_weights else val
+
+
+def model_likelihood_grad_fd(params, dict_data, num_Vbin, Rzphi_n_tot, sigma_amplify=1.0,
+                             schedule=None, nnls_maxiter=200, W_fixed=None, h=5e-5):
+    """The gradient jax.grad takes of model_likelihood, by central differences.
+
+    jax.grad of likelihood.py:96-244 differentiates the piece of the
+    piecewise-smooth map on which theta lies: accept / reject decisions, bin
+    membership and validity flags are constants (integrates.py:103-111,
+    149-159, 210-236), and so are the branches of the error scale's max and
+    of the controller's clips; the NNLS weights are stop_gradient'ed
+    (nnls.py:165).  So the base run records those decisions and every
+    perturbed run replays them with the base (or the given) weights; the
+    fourth-order central difference of that smooth map is the same derivative
+    up to O(h^4) and the map's roundoff (~1e-10 relative, from the nested
+    finite differences of the Jeans moments, jeans.py:37-60) / h.
+
+    Returns (logL at theta, gradient (12,), W).
+    """
+    theta = np.asarray(params, dtype=np.float64)
+    tape = {'mode': 'record'}
+    f0, W = model_likelihood(theta, dict_data, num_Vbin, Rzphi_n_tot, sigma_amplify, schedule,
+                             nnls_maxiter, tape=tape, W_fixed=W_fixed, return_weights=True)
+    tape['mode'] = 'replay'
+    g = np.zeros(theta.size)
+    for i in range(theta.size):
+        e = np.zeros(theta.size)
+        e[i] = h
+        f = {k: model_likelihood(theta + k * e, dict_data, num_Vbin, Rzphi_n_tot, sigma_amplify,
+                                 schedule, nnls_maxiter, tape=tape, W_fixed=W) for k in (-2, -1, 1, 2)}
+        g[i] = (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * h)
+    return f0, g, W

@@ -68,6 +68,18 @@ _SIGS = [
     ('smx_build_library', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_build_library_ckpt', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P]),
+    ('smx_library_vjp_workspace_bytes', _I64, [_I64, _I32]),
+    ('smx_library_vjp', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_initial_conditions_jac', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, _I32, _I32, ctypes.c_double, _P, _P, _P]),
+    ('smx_density_target_jac', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double,
+      ctypes.c_double, _P, _I64, _P, _P]),
     ('smx_build_library_traj', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -146,7 +158,7 @@ def dev(a, dtype=torch.float64, device='cuda'):
 
 
 STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps', 'density',
-          'admm_product')
+          'admm_product', 'adjoint', 'jacobian')
 
 
 def timing_enable(on=True):

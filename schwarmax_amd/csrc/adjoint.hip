@@ -1,0 +1,563 @@
+// Reverse-mode derivative of the model evaluation up to the orbit library,
+// the derivative jax.grad takes of the reference (model.py:70-258 through
+// integrates.py:37-300): bin membership, accept / reject decisions and
+// validity flags held fixed, everything else differentiated, including the
+// step-size controller (integrates.py:100-111), the Gauss-Hermite weights
+// (ghmoments.py:6-36), the T normalisation and the realization average.
+//
+//  * library_vjp_kernel: one integration per lane.  The forward pass stored
+//    the state (y, dt, t) before every ckpt_every-th attempt; each segment is
+//    replayed from its checkpoint, storing the attempts' stage slopes, and
+//    then swept backwards.  Output: cotangents of the integration's initial
+//    state and dt0, of Omega, of the sky rotation's third row and of the
+//    potential fields.
+//  * The force's vector-Jacobian product uses the symmetry of the Hessian of
+//    Phi (J^T w = J w, one directional derivative) and forward-mode tangents
+//    for the component parameters, each component with its own dual size.
+//  * initial_conditions_jac / density_target_jac: forward-mode Jacobians of
+//    the Jeans ICs, time scales and density target with respect to the
+//    potential fields (dual numbers seeded a few fields per pass).
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+#include "common.hpp"
+#include "potential.hpp"
+#include "binning.hpp"
+#include "integrator.hpp"
+#include "ics.hpp"
+#include "timing.hpp"
+#include "../../include/smx.h"
+
+namespace smx {
+
+static_assert(PF_COUNT == SMX_POT_FIELDS, "field count mismatch");
+
+// ---------------------------------------------------------------------------
+// Force VJP: k = deriv(y) (integrates.py:8-27).  For cotangent w of k, add
+// (dk/dy)^T w to yb, dk/dOmega . w to omb and dk/dfield . w to fb.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void seed_xyz(const double* y, const double* dir, Dual<N>& X, Dual<N>& Y,
+                                         Dual<N>& Z) {
+  X = dconst<N>(y[0]);
+  Y = dconst<N>(y[1]);
+  Z = dconst<N>(y[2]);
+  X.d[0] = dir[0];
+  Y.d[0] = dir[1];
+  Z.d[0] = dir[2];
+}
+
+template <int N>
+__device__ __forceinline__ void take_component(double sgn, const double* wv, const Dual<N>& gx,
+                                               const Dual<N>& gy, const Dual<N>& gz, double* yb,
+                                               double* fb, const int* slot_field) {
+  // acceleration of this component = sgn * grad; Hessian symmetric
+  yb[0] += sgn * gx.d[0];
+  yb[1] += sgn * gy.d[0];
+  yb[2] += sgn * gz.d[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k)
+    fb[slot_field[k]] += sgn * ((wv[0] * gx.d[k] + wv[1] * gy.d[k]) + wv[2] * gz.d[k]);
+}
+
+__device__ __noinline__ void deriv_vjp(const smx_potential& p, double Om, const double* y,
+                                       const double* w, double* yb, double& omb, double* fb) {
+  // rotating-frame terms: k0 = y3 + Om y1, k1 = y4 - Om y0, k2 = y5,
+  // k3 = ax + Om y4, k4 = ay - Om y3, k5 = az
+  yb[0] += -Om * w[1];
+  yb[1] += Om * w[0];
+  yb[3] += w[0] - Om * w[4];
+  yb[4] += w[1] + Om * w[3];
+  yb[5] += w[2];
+  omb += (w[0] * y[1] - w[1] * y[0]) + (w[3] * y[4] - w[4] * y[3]);
+  const double wv[3] = {w[3], w[4], w[5]};
+  {  // NFW: slots [direction, mGM, Rs]; a = -grad
+    typedef Dual<3> D;
+    D X, Y, Z, gx, gy, gz;
+    seed_xyz(y, wv, X, Y, Z);
+    const D m = dseed<3>(p.nfw_mGM, 1), rs = dseed<3>(p.nfw_Rs, 2);
+    nfw_grad_t(p, m, rs, X, Y, Z, gx, gy, gz);
+    const int sf[3] = {0, PF_NFW_MGM, PF_NFW_RS};
+    take_component(-1.0, wv, gx, gy, gz, yb, fb, sf);
+  }
+  {  // Miyamoto-Nagai: [direction, mGM, Rs, Hs]; a = -grad
+    typedef Dual<4> D;
+    D X, Y, Z, gx, gy, gz;
+    seed_xyz(y, wv, X, Y, Z);
+    const D m = dseed<4>(p.mn_mGM, 1), rs = dseed<4>(p.mn_Rs, 2), hs = dseed<4>(p.mn_Hs, 3);
+    mn_grad_t(p, m, rs, hs, X, Y, Z, gx, gy, gz);
+    const int sf[4] = {0, PF_MN_MGM, PF_MN_RS, PF_MN_HS};
+    take_component(-1.0, wv, gx, gy, gz, yb, fb, sf);
+  }
+  {  // T3 bar: [direction, M, L, a, b]; a = +G grad(ph)
+    typedef Dual<5> D;
+    D X, Y, Z, gx, gy, gz;
+    seed_xyz(y, wv, X, Y, Z);
+    const D m = dseed<5>(p.t3_M, 1), L = dseed<5>(p.t3_L, 2), a = dseed<5>(p.t3_a, 3),
+            b = dseed<5>(p.t3_b, 4);
+    t3_grad_t(p, m, L, a, b, X, Y, Z, gx, gy, gz);
+    const int sf[5] = {0, PF_T3_M, PF_T3_L, PF_T3_A, PF_T3_B};
+    take_component(p.G, wv, gx, gy, gz, yb, fb, sf);
+  }
+  {  // V4 bulge: [direction, M]; a = +G grad(ph)
+    typedef Dual<2> D;
+    D X, Y, Z, gx, gy, gz;
+    seed_xyz(y, wv, X, Y, Z);
+    const D m = dseed<2>(p.v4_M, 1);
+    v4_grad_t(p, m, X, Y, Z, gx, gy, gz);
+    const int sf[2] = {0, PF_V4_M};
+    take_component(p.G, wv, gx, gy, gz, yb, fb, sf);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binning VJP of one accepted sample y with weight w (bin_sample's adjoint).
+// gr is the library cotangent row of the sample's column.  Returns dL/dw
+// (unscaled); adds sc * dL/dy to ynb (velocities only: positions enter
+// through bin membership alone) and sc * dL/d(rot[6..8]) to rotb.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bin_vjp(const smx_grids& g, const int32_t* __restrict__ bm,
+                          const double* __restrict__ v0, const double* __restrict__ sd,
+                          const double* __restrict__ npb, const double* __restrict__ gr,
+                          const double* y, double w, double sc, int nsym, int P, double* ynb,
+                          double* rotb) {
+  const int B = g.num_vbin;
+  SampleProj sp;
+  project(g, y, nsym > 2, sp);
+  double dw = 0.0;
+  for (int c = 0; c < nsym; ++c) {
+    double zs, phi, xr, yr, vlos;
+    copy_coords(sp, c, zs, phi, xr, yr, vlos);
+    const int cell = rzphi_cell(g, sp.R, zs * y[2], phi);
+    if (cell >= 0) dw += gr[cell];
+    const int a = aperture(g, bm, xr, yr);
+    if (a >= 0) {
+      double gw, H[5];
+      gh_terms(vlos, v0[a], sd[a], gw, H);
+      // library rows [Rzphi | xy | h1 | h2 | h3 | h4 | h0]
+      const double gh[5] = {gr[P + 5 * B + a], gr[P + B + a], gr[P + 2 * B + a],
+                            gr[P + 3 * B + a], gr[P + 4 * B + a]};
+      double sH = 0.0, sdH = 0.0;
+      const double u = (vlos - v0[a]) / sd[a];
+      // d/du (g H_k) = g (H_k' - u H_k), H_k' = sqrt(2k) H_{k-1}
+      const double dH[5] = {0.0, kSq2 * H[0], 2.0 * H[1], 2.4494897427831781 * H[2],
+                            2.8284271247461901 * H[3]};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        sH += gh[k] * H[k];
+        sdH += gh[k] * (dH[k] - u * H[k]);
+      }
+      dw += gr[P + a] / (npb[a] * g.area_pixel + kEps) + gw * sH;
+      // v_los = (sxy (M6 vx + M7 vy) + sz M8 vz) KPCGYR_TO_KMS
+      const double dv = sc * w * gw * sdH / sd[a] * kKpcGyrToKms;
+      const double sxy = (c >= 2) ? -1.0 : 1.0;
+      const double szz = (c & 1) ? -1.0 : 1.0;
+      ynb[3] += dv * sxy * g.rot[6];
+      ynb[4] += dv * sxy * g.rot[7];
+      ynb[5] += dv * szz * g.rot[8];
+      rotb[0] += dv * sxy * y[3];
+      rotb[1] += dv * sxy * y[4];
+      rotb[2] += dv * szz * y[5];
+    }
+  }
+  return dw;
+}
+
+// ---------------------------------------------------------------------------
+// Per-integration scale and T cotangent.  lib[c] = wt_c sum_r valid_r X_r,
+// X_r = slab_r * c_e / T_r (c_e = 1, or 1/(num_per_bin area + eps) for the
+// surface-density rows), wt_c = 1/(sum_r valid_r + 0.1).  With gbar the
+// library cotangent: scale_r = wt_c valid_r / T_r multiplies every binned
+// contribution, and dL/dT_r = -scale_r / T_r * sum_e gbar[c][e] c_e slab_r[e].
+// One wave per integration.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void tdot_kernel(const double* __restrict__ ws, int64_t n_int,
+                                                   int r, int P, int B,
+                                                   const double* __restrict__ valid,
+                                                   const double* __restrict__ T_int,
+                                                   const double* __restrict__ npb, double area,
+                                                   const double* __restrict__ gbar,
+                                                   double* __restrict__ scale,
+                                                   double* __restrict__ tbar) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n_int) return;
+  const int64_t c = i / r;
+  double nv = 0.0;
+  for (int k = 0; k < r; ++k) nv += valid[c * r + k];
+  const double sc = valid[i] > 0.5 ? (1.0 / (nv + 0.1)) / T_int[i] : 0.0;
+  const int64_t rows = (int64_t)P + 6 * (int64_t)B;
+  const double* gr = gbar + c * rows;
+  const double* rz = ws + i * slab_stride(P, B);
+  const double* ap = rz + slab_rz_len(P);
+  double dot = 0.0;
+  if (sc != 0.0) {
+    for (int64_t e = lane; e < rows; e += 64) {
+      double v;
+      if (e < P) {
+        v = rz[e];
+      } else {
+        const int64_t q = e - P;
+        const int64_t blk = q / B, a = q - blk * B;
+        const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
+        v = ap[8 * a + comp];
+        if (blk == 0) v = v / (npb[a] * area + kEps);
+      }
+      dot += gr[e] * v;
+    }
+  }
+  dot = wave_sum(dot);
+  if (lane == 0) {
+    scale[i] = sc;
+    tbar[i] = -sc * dot / T_int[i];
+  }
+}
+
+// per-lane replay buffer: attempt a, component j of [y(6) dt k1(6) k2(6) k3(6) k4(6)]
+constexpr int SEG_W = 31;
+
+__device__ __forceinline__ double& segv(double* seg, int64_t n_int, int64_t i, int a, int j) {
+  return seg[((int64_t)a * SEG_W + j) * n_int + i];
+}
+
+__global__ __launch_bounds__(64) void library_vjp_kernel(
+    smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
+    const double* __restrict__ dt0, int64_t n_int, int r, const int32_t* __restrict__ bm,
+    const double* __restrict__ npb, const double* __restrict__ v0, const double* __restrict__ sd,
+    const double* __restrict__ gbar, const double* __restrict__ scale,
+    const double* __restrict__ tbar, const double* __restrict__ ckpt, int every,
+    double* __restrict__ seg, double* __restrict__ ybar0, double* __restrict__ dtbar0,
+    double* __restrict__ fbar, double* __restrict__ ombar, double* __restrict__ rotbar) {
+  // Every lane of a wave runs the same sequence (lanes past n_int redo the
+  // last integration, invalid integrations run with scale 0 and yield 0), so
+  // deriv_vjp is only ever called with all lanes active.
+  const int64_t i_raw = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool live = i_raw < n_int;
+  const int64_t i = live ? i_raw : n_int - 1;
+  const int P = g.nR * g.nz * g.nphi;
+  const int B = g.num_vbin;
+  const int nsym = g.symmetry ? 4 : 1;
+  const double Om = s.Omega;
+  const double sc = scale[i];
+  double yb[6] = {0, 0, 0, 0, 0, 0}, k1b[6] = {0, 0, 0, 0, 0, 0};
+  double dtb = 0.0, omb = 0.0;
+  double fb[PF_COUNT];
+#pragma unroll
+  for (int f = 0; f < PF_COUNT; ++f) fb[f] = 0.0;
+  double rotb[3] = {0.0, 0.0, 0.0};
+  {
+    const double Tb = tbar[i];
+    const double* gr = gbar + (i / r) * ((int64_t)P + 6 * (int64_t)B);
+    const int nseg = (s.N_max + every - 1) / every;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+      const int a0 = sg * every;
+      const int na = (s.N_max - a0 < every) ? s.N_max - a0 : every;
+      // replay the segment from its checkpoint
+      {
+        const double* c = ckpt + ((int64_t)sg * n_int + i) * 8;
+        double y[6], k1[6], dt = c[6], t = c[7];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) y[j] = c[j];
+        deriv(p, Om, y, k1);
+        for (int a = 0; a < na; ++a) {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            if (live) {
+              segv(seg, n_int, i, a, j) = y[j];
+              segv(seg, n_int, i, a, 7 + j) = k1[j];
+            }
+          }
+          if (live) segv(seg, n_int, i, a, 6) = dt;
+          double k2[6], k3[6], k4[6];
+          bs3_attempt_stages(p, s, t, y, k1, dt, k2, k3, k4);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            if (live) {
+              segv(seg, n_int, i, a, 13 + j) = k2[j];
+              segv(seg, n_int, i, a, 19 + j) = k3[j];
+              segv(seg, n_int, i, a, 25 + j) = k4[j];
+            }
+          }
+        }
+      }
+      // reverse sweep
+      for (int a = na - 1; a >= 0; --a) {
+        double y[6], k1[6], k2[6], k3[6], k4[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          y[j] = segv(seg, n_int, i, a, j);
+          k1[j] = segv(seg, n_int, i, a, 7 + j);
+          k2[j] = segv(seg, n_int, i, a, 13 + j);
+          k3[j] = segv(seg, n_int, i, a, 19 + j);
+          k4[j] = segv(seg, n_int, i, a, 25 + j);
+        }
+        const double dt = segv(seg, n_int, i, a, 6);
+        double yn[6], e3[6], q[6], scl[6];
+        double es = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          yn[j] = y[j] + dt * ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * k3[j]);
+          e3[j] = (-5.0 / 72.0 * k1[j] + 1.0 / 12.0 * k2[j]) + 1.0 / 9.0 * k3[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          const double e = dt * (e3[j] + -1.0 / 8.0 * k4[j]);
+          scl[j] = s.atol + s.rtol * nmax(fabs(y[j]), fabs(yn[j]));
+          q[j] = e / scl[j];
+          es += q[j] * q[j];
+        }
+        const double err = sqrt(es / 6.0);
+        const bool accept = err <= 1.0;
+        const double sf_raw = 0.9 * pow(nmax(err, 1e-10), -1.0 / 3.0);
+        const double sf = nmin(nmax(sf_raw, 0.2), 5.0);
+        const double dtn_raw = dt * sf;
+
+        double ybA[6] = {0, 0, 0, 0, 0, 0}, ynb[6] = {0, 0, 0, 0, 0, 0};
+        double k2b[6], k3b[6], k4b[6], k1l[6];
+        double dtA = 0.0;
+        if (accept) {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) ynb[j] = yb[j];
+          // the sample y_{a+1} = yn binned with weight dt; T += nsym dt
+          const double dw = bin_vjp(g, bm, v0, sd, npb, gr, yn, dt, sc, nsym, P, ynb, rotb);
+          dtA += sc * dw + nsym * Tb;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) ybA[j] = yb[j];
+        }
+        // step-size controller: dt_{a+1} = clip(dt clip(0.9 max(err,1e-10)^(-1/3), 0.2, 5), dt_min, dt_max)
+        const double prodb = (dtn_raw > s.dt_min && dtn_raw < s.dt_max) ? dtb : 0.0;
+        dtA += prodb * sf;
+        const double sfb = (sf_raw > 0.2 && sf_raw < 5.0) ? prodb * dt : 0.0;
+        const double eb = (err > 1e-10) ? sfb * (-1.0 / 3.0) * sf_raw / err : 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          double evb = 0.0;
+          if (err > 0.0 && eb != 0.0) {
+            const double qb = eb * q[j] / (6.0 * err);
+            evb = qb / scl[j];
+            const double scb = -qb * q[j] / scl[j];
+            const double ay = fabs(y[j]), an = fabs(yn[j]);
+            const double sy = y[j] > 0.0 ? 1.0 : (y[j] < 0.0 ? -1.0 : 0.0);
+            const double sn = yn[j] > 0.0 ? 1.0 : (yn[j] < 0.0 ? -1.0 : 0.0);
+            if (ay > an) {
+              ybA[j] += s.rtol * scb * sy;
+            } else if (an > ay) {
+              ynb[j] += s.rtol * scb * sn;
+            } else {
+              ybA[j] += 0.5 * s.rtol * scb * sy;
+              ynb[j] += 0.5 * s.rtol * scb * sn;
+            }
+          }
+          // err_j = dt (e3_j - k4_j / 8)
+          dtA += evb * (e3[j] + -1.0 / 8.0 * k4[j]);
+          k1l[j] = dt * (-5.0 / 72.0) * evb;
+          k2b[j] = dt * (1.0 / 12.0) * evb;
+          k3b[j] = dt * (1.0 / 9.0) * evb;
+          k4b[j] = dt * (-1.0 / 8.0) * evb;
+          // FSAL: k1_{a+1} = accept ? k4 : k1
+          if (accept) k4b[j] += k1b[j];
+          else k1l[j] += k1b[j];
+        }
+        // k4 = f(yn)
+        deriv_vjp(p, Om, yn, k4b, ynb, omb, fb);
+        // yn = y + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          ybA[j] += ynb[j];
+          k1l[j] += dt * (2.0 / 9.0) * ynb[j];
+          k2b[j] += dt * (1.0 / 3.0) * ynb[j];
+          k3b[j] += dt * (4.0 / 9.0) * ynb[j];
+          dtA += ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * k3[j]) * ynb[j];
+        }
+        // k3 = f(y + 0.75 dt k2)
+        {
+          double ys[6], ysb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int j = 0; j < 6; ++j) ys[j] = y[j] + (0.75 * dt) * k2[j];
+          deriv_vjp(p, Om, ys, k3b, ysb, omb, fb);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            ybA[j] += ysb[j];
+            k2b[j] += (0.75 * dt) * ysb[j];
+            dtA += 0.75 * k2[j] * ysb[j];
+          }
+        }
+        // k2 = f(y + 0.5 dt k1)
+        {
+          double ys[6], ysb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int j = 0; j < 6; ++j) ys[j] = y[j] + (0.5 * dt) * k1[j];
+          deriv_vjp(p, Om, ys, k2b, ysb, omb, fb);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            ybA[j] += ysb[j];
+            k1l[j] += (0.5 * dt) * ysb[j];
+            dtA += 0.5 * k1[j] * ysb[j];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          yb[j] = ybA[j];
+          k1b[j] = k1l[j];
+        }
+        dtb = dtA;
+      }
+    }
+    // k1_0 = f(y_0)
+    double y0[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) y0[j] = wb[6 * i + j];
+    deriv_vjp(p, Om, y0, k1b, yb, omb, fb);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) ybar0[6 * i + j] = yb[j];
+  dtbar0[i] = dtb;
+#pragma unroll
+  for (int f = 0; f < PF_COUNT; ++f) fbar[i * PF_COUNT + f] = fb[f];
+  ombar[i] = omb;
+  for (int j = 0; j < 3; ++j) rotbar[3 * i + j] = rotb[j];
+}
+
+// ---------------------------------------------------------------------------
+// Field Jacobians of the initial conditions and the density target.
+// ---------------------------------------------------------------------------
+template <int NJ>
+__global__ __launch_bounds__(256) void ic_jac_kernel(smx_potential p, const double* __restrict__ w0,
+                                                     const double* __restrict__ normals,
+                                                     const double* __restrict__ uniforms, int64_t n,
+                                                     int r, int r_total, double N_step, int f0,
+                                                     double* __restrict__ dwb,
+                                                     double* __restrict__ ddt0) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  StarIC<Dual<NJ>> o;
+  star_ic<Dual<NJ>>(p, fields_seeded<NJ>(p, f0), w0, normals, i, lane, o);
+  if (lane != 0) return;
+  for (int k = 0; k < r; ++k) {
+    const int64_t q = i * r + k;
+    for (int j = 0; j < 6; ++j) {
+      const double u = uniforms[(i * r_total + k) * 6 + j] - 0.5;
+#pragma unroll
+      for (int t = 0; t < NJ; ++t) {
+        if (f0 + t >= PF_COUNT) continue;
+        const double ds = (j < 3) ? 0.0 : o.vsc.d[t];
+        dwb[(6 * q + j) * PF_COUNT + f0 + t] = o.wn[j].d[t] + u * ds;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NJ; ++t)
+      if (f0 + t < PF_COUNT) ddt0[q * PF_COUNT + f0 + t] = o.T_orb.d[t] / N_step;
+  }
+}
+
+template <int NJ>
+__global__ __launch_bounds__(256) void density_jac_kernel(
+    smx_potential p, const double* __restrict__ Rg, const double* __restrict__ zg,
+    const double* __restrict__ pg, int64_t P, double dR, double dz, double dphi,
+    const double* __restrict__ smp, int64_t S, int f0, double* __restrict__ jac) {
+  __shared__ double red[4][NJ];
+  const int64_t c = blockIdx.x;
+  if (c >= P) return;
+  Dual<NJ> acc = cell_mass_part<Dual<NJ>>(p, fields_seeded<NJ>(p, f0), Rg[c], zg[c], pg[c], dR, dz,
+                                          dphi, smp, S, threadIdx.x, blockDim.x);
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0)
+    for (int t = 0; t < NJ; ++t) red[threadIdx.x >> 6][t] = acc.d[t];
+  __syncthreads();
+  if (threadIdx.x < NJ && f0 + (int)threadIdx.x < PF_COUNT) {
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w][threadIdx.x];
+    jac[c * PF_COUNT + f0 + threadIdx.x] = ((tot / static_cast<double>(S)) * dR * dz) * dphi;
+  }
+}
+
+constexpr int JAC_SLOTS = 4;
+
+}  // namespace smx
+
+using namespace smx;
+
+extern "C" {
+
+int64_t smx_library_vjp_workspace_bytes(int64_t n_int, int32_t ckpt_every) {
+  if (n_int < 0 || ckpt_every < 1) return -1;
+  return (int64_t)sizeof(double) * (n_int * SEG_W * (int64_t)ckpt_every + 2 * n_int) + 256;
+}
+
+int smx_library_vjp(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                    const double* wb, const double* dt0, int64_t n, int32_t r,
+                    const int32_t* bin_mapping, const double* num_per_bin, const double* v0,
+                    const double* sdisp, const void* fwd_workspace, const double* valid,
+                    const double* T_int, const double* ckpt, int32_t ckpt_every,
+                    const double* lib_bar, void* workspace, double* wb_bar, double* dt0_bar,
+                    double* field_bar, double* omega_bar, double* rot_bar, void* stream) {
+  SMX_REQUIRE(p && g && s && wb && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
+                  fwd_workspace && valid && T_int && ckpt && lib_bar && workspace && wb_bar &&
+                  dt0_bar && field_bar && omega_bar && rot_bar,
+              "smx_library_vjp: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0 && ckpt_every >= 1, "smx_library_vjp: bad sizes");
+  SMX_REQUIRE(g->nR > 0 && g->nz > 0 && g->nphi > 0 && g->num_vbin > 0, "smx_library_vjp: bad grids");
+  if (n == 0) return 0;
+  const int64_t n_int = n * r;
+  const int P = g->nR * g->nz * g->nphi;
+  hipStream_t st = as_stream(stream);
+  double* seg = static_cast<double*>(workspace);
+  double* scale = seg + n_int * SEG_W * (int64_t)ckpt_every;
+  double* tbar = scale + n_int;
+  stage_begin(ST_ADJOINT, st);
+  hipLaunchKernelGGL(tdot_kernel, dim3(blocks_for(n_int, 4)), dim3(256), 0, st,
+                     static_cast<const double*>(fwd_workspace), n_int, (int)r, P, g->num_vbin, valid,
+                     T_int, num_per_bin, g->area_pixel, lib_bar, scale, tbar);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(library_vjp_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p, *g, *s,
+                     wb, dt0, n_int, (int)r, bin_mapping, num_per_bin, v0, sdisp, lib_bar, scale,
+                     tbar, ckpt, (int)ckpt_every, seg, wb_bar, dt0_bar, field_bar, omega_bar,
+                     rot_bar);
+  SMX_LAUNCHED();
+  stage_end(ST_ADJOINT, st, 2);
+  return 0;
+}
+
+int smx_initial_conditions_jac(const smx_potential* p, const double* w0, const double* normals,
+                               const double* uniforms, int64_t n, int32_t r, int32_t r_total,
+                               double N_step, double* wb_jac, double* dt0_jac, void* stream) {
+  SMX_REQUIRE(p && w0 && normals && uniforms && wb_jac && dt0_jac,
+              "smx_initial_conditions_jac: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1 && r_total >= r && N_step > 0.0,
+              "smx_initial_conditions_jac: bad sizes");
+  if (n == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const int waves_per_block = 4;
+  stage_begin(ST_JACOBIAN, st);
+  for (int f0 = 0; f0 < PF_COUNT; f0 += JAC_SLOTS) {
+    hipLaunchKernelGGL(ic_jac_kernel<JAC_SLOTS>, dim3(blocks_for(n, waves_per_block)),
+                       dim3(64 * waves_per_block), 0, st, *p, w0, normals, uniforms, n, (int)r,
+                       (int)r_total, N_step, f0, wb_jac, dt0_jac);
+    SMX_LAUNCHED();
+  }
+  stage_end(ST_JACOBIAN, st, 3);
+  return 0;
+}
+
+int smx_density_target_jac(const smx_potential* p, const double* R_grid, const double* z_grid,
+                           const double* phi_grid, int64_t P, double dR, double dz, double dphi,
+                           const double* sobol, int64_t S, double* jac, void* stream) {
+  SMX_REQUIRE(p && R_grid && z_grid && phi_grid && sobol && jac,
+              "smx_density_target_jac: null pointer");
+  SMX_REQUIRE(P >= 0 && S > 0, "smx_density_target_jac: bad sizes");
+  if (P == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  stage_begin(ST_JACOBIAN, st);
+  for (int f0 = 0; f0 < PF_COUNT; f0 += JAC_SLOTS) {
+    hipLaunchKernelGGL(density_jac_kernel<JAC_SLOTS>, dim3((unsigned)P), dim3(256), 0, st, *p,
+                       R_grid, z_grid, phi_grid, P, dR, dz, dphi, sobol, S, f0, jac);
+    SMX_LAUNCHED();
+  }
+  stage_end(ST_JACOBIAN, st, 3);
+  return 0;
+}
+
+}  // extern "C"

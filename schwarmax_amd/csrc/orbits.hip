@@ -11,6 +11,8 @@
 #include "common.hpp"
 #include "potential.hpp"
 #include "binning.hpp"
+#include "integrator.hpp"
+#include "ics.hpp"
 #include "timing.hpp"
 
 namespace smx {
@@ -60,41 +62,6 @@ __global__ void density_eval_kernel(smx_potential p, const double* __restrict__ 
 // Jeans initial conditions (jeans.py:9-94) + jitter / time scales
 // (model.py:117-169).  One 64-lane wavefront per star.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// trapezoid weights for numpy-style linspace(lo, 10, npts): point i, interval
-// i -> i+1 has width pts[i+1]-pts[i]
-__device__ __forceinline__ double lin_pt(double lo, double hi, double step, int i, int npts) {
-  return (i == npts - 1) ? hi : (static_cast<double>(i) * step + lo);
-}
-
-__device__ __forceinline__ double dphi_dz(const smx_potential& p, double x, double y, double z) {
-  const double d = 5e-3;
-  return (phi_total(p, x, y, z + d) - phi_total(p, x, y, z - d)) / (2.0 * d);
-}
-
-// integral_{|z*|}^{10} rho(x, y, z') dPhi/dz(x, y, z') dz' by the trapezoid
-// rule on npts points, this lane's share; summed over the wave by the caller.
-__device__ double pressure_part(const smx_potential& p, double x, double y, double zlo, int npts,
-                                int lane) {
-  const double hi = 10.0;
-  const double step = (hi - zlo) / static_cast<double>(npts - 1);
-  double acc = 0.0;
-  for (int i = lane; i < npts; i += 64) {
-    const double zi = lin_pt(zlo, hi, step, i, npts);
-    const double f = rho_total(p, x, y, zi) * dphi_dz(p, x, y, zi);
-    double w = 0.0;
-    if (i > 0) w += zi - lin_pt(zlo, hi, step, i - 1, npts);
-    if (i < npts - 1) w += lin_pt(zlo, hi, step, i + 1, npts) - zi;
-    acc += 0.5 * w * f;
-  }
-  return acc;
-}
-
 __global__ __launch_bounds__(256) void initial_conditions_kernel(
     smx_potential p, const double* __restrict__ w0, const double* __restrict__ normals,
     const double* __restrict__ uniforms, int64_t n, int r, int r_total, double N_step,
@@ -102,156 +69,30 @@ __global__ __launch_bounds__(256) void initial_conditions_kernel(
   const int lane = threadIdx.x & 63;
   const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= n) return;   // whole wave exits together
-  const double xs = w0[3 * i], ys = w0[3 * i + 1], zs = w0[3 * i + 2];
-  const double Rs = sqrt(xs * xs + ys * ys);
-  const double zlo = fabs(zs);
-  const int npts = 1000;
-  const double dR = 5e-3;
-  const double I0 = wave_sum(pressure_part(p, xs, ys, zlo, npts, lane));
-  const double Ip = wave_sum(pressure_part(p, Rs + dR, 0.0, zlo, npts, lane));
-  const double Im = wave_sum(pressure_part(p, Rs - dR, 0.0, zlo, npts, lane));
+  StarIC<double> o;
+  star_ic<double>(p, fields_const<double>(p), w0, normals, i, lane, o);
   if (lane != 0) return;
-  const double b = 1.0;   // anisotropy_b (jeans.py:86)
-  const double nu = rho_total(p, xs, ys, zs);
-  const double sz2 = nmax((1.0 / nu) * I0, 0.0);
-  const double sz = sqrt(sz2);
-  const double sR2 = b * sz2;
-  const double sR = sqrt(sR2);
-  const double dnu = b * (Ip - Im) / (2.0 * dR);
-  const double d = 5e-3;
-  const double dphidR = (phi_total(p, Rs + d, 0.0, zs) - phi_total(p, Rs - d, 0.0, zs)) / (2.0 * d);
-  const double vtot2 = (sR2 + (Rs / nu) * dnu) + Rs * dphidR;
-  const double sphi = sR;
-  double vmean = sqrt(nmax(vtot2 - sphi * sphi, 0.0));
-  double vR_s = sR, vz_s = sz, vp_s = sphi;
-  if (nu <= 0.0) { vmean = 0.0; vR_s = 0.0; vz_s = 0.0; vp_s = 0.0; }
-  const double g1 = normals[3 * i], g2 = normals[3 * i + 1], g3 = normals[3 * i + 2];
-  const double vR = g1 * vR_s, vz = g2 * vz_s, vphi = vmean + g3 * vp_s;
-  // clockwise cylindrical -> Cartesian (utils.py:144-162)
-  const double phi = atan2(ys, xs);
-  const double c = cos(phi), s = sin(phi);
-  const double wn[6] = {Rs * c, Rs * s, zs, vR * c + vphi * s, vR * s - vphi * c, vz};
-  // rotation curve and orbital time scale at the new ICs (utils.py:182-273)
-  const double R = sqrt(wn[0] * wn[0] + wn[1] * wn[1]);
-  const double dRt = 1e-3;
-  const double Rsafe = fmax(fabs(R), 2.0 * dRt);
-  const double dd = (phi_total(p, Rsafe + dRt, 0.0, zs) - phi_total(p, Rsafe - dRt, 0.0, zs)) / (2.0 * dRt);
-  const double vc = sqrt(nmax(Rsafe * dd, 0.0));
-  const double om = sqrt(nmax(fabs(dd) / Rsafe, 1e-20));
-  const double T_orb = (2.0 * kPi) / om;
-  const double vsc = nmin(nmax(0.1 * vc, 1.0), 15.0);
-  const double scale[6] = {0.1, 0.1, 0.1, vsc, vsc, vsc};
+  const double scale[6] = {0.1, 0.1, 0.1, o.vsc, o.vsc, o.vsc};
   for (int k = 0; k < r; ++k) {
-    const int64_t o = i * r + k;
+    const int64_t q = i * r + k;
     for (int j = 0; j < 6; ++j)
-      wb[6 * o + j] = wn[j] + (uniforms[(i * r_total + k) * 6 + j] - 0.5) * scale[j];
-    T_total[o] = T_orb * N_dyn;
-    dt0[o] = T_orb / N_step;
+      wb[6 * q + j] = o.wn[j] + (uniforms[(i * r_total + k) * 6 + j] - 0.5) * scale[j];
+    T_total[q] = o.T_orb * N_dyn;
+    dt0[q] = o.T_orb / N_step;
   }
 }
 
 // ---------------------------------------------------------------------------
-// BS3 integrator (integrates.py:8-112)
+// Fused BS3 integration + binning (integrates.py:37-206): one integration per
+// lane, samples accumulated into the lane's own slab.  With ckpt, the state
+// (y, dt, t) before every ckpt_every-th attempt is stored for the adjoint.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void deriv(const smx_potential& p, double Om, const double* y, double* k) {
-  double ax, ay, az;
-  accel_fast(p, y[0], y[1], y[2], ax, ay, az);
-  k[0] = y[3] + Om * y[1];
-  k[1] = y[4] - Om * y[0];
-  k[2] = y[5];
-  k[3] = ax + Om * y[4];
-  k[4] = ay - Om * y[3];
-  k[5] = az;
-}
-
-__device__ __forceinline__ double jacobi(const smx_potential& p, double Om, const double* y) {
-  const double ek = 0.5 * ((y[3] * y[3] + y[4] * y[4]) + y[5] * y[5]);
-  return (ek + phi_total(p, y[0], y[1], y[2])) - Om * (y[0] * y[4] - y[1] * y[3]);
-}
-
-// One attempt.  On return y holds y_out, k1 the FSAL slope, dt the next step;
-// returns the step length used when accepted, 0 otherwise.  The stage
-// vectors are folded as soon as they are complete (the error sum's first
-// three terms before k4 is formed), keeping the reference's summation order.
-__device__ __forceinline__ double bs3_attempt(const smx_potential& p, const smx_schedule& s,
-                                              double& t, double* y, double* k1, double& dt) {
-  const double Om = s.Omega;
-  double yt[6], k2[6], kk[6], yn[6], e3[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.5 * dt) * k1[j];
-  deriv(p, Om, yt, k2);
-#pragma unroll
-  for (int j = 0; j < 6; ++j) yt[j] = y[j] + (0.75 * dt) * k2[j];
-  deriv(p, Om, yt, kk);                                  // k3
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    yn[j] = y[j] + dt * ((2.0 / 9.0 * k1[j] + 1.0 / 3.0 * k2[j]) + 4.0 / 9.0 * kk[j]);
-    e3[j] = (-5.0 / 72.0 * k1[j] + 1.0 / 12.0 * k2[j]) + 1.0 / 9.0 * kk[j];
-  }
-  deriv(p, Om, yn, kk);                                  // k4
-  double es = 0.0;
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const double e = dt * (e3[j] + -1.0 / 8.0 * kk[j]);
-    const double sc = s.atol + s.rtol * nmax(fabs(y[j]), fabs(yn[j]));
-    const double q = e / sc;
-    es += q * q;
-  }
-  const double err = sqrt(es / 6.0);
-  const bool accept = err <= 1.0;
-  const double used = accept ? dt : 0.0;
-  if (accept) {
-    t = t + dt;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      y[j] = yn[j];
-      k1[j] = kk[j];
-    }
-  }
-  double sf = 0.9 * pow(nmax(err, 1e-10), -1.0 / 3.0);
-  sf = nmin(nmax(sf, 0.2), 5.0);
-  dt = nmin(nmax(dt * sf, s.dt_min), s.dt_max);
-  return used;
-}
-
-// per-integration accumulator slab: P cells (padded to 8) then B records of
-// 8 doubles [dt, g dt H0, ..., g dt H4, -, -]
-__host__ __device__ __forceinline__ int64_t slab_rz_len(int P) { return (P + 7) & ~7; }
-__host__ __device__ __forceinline__ int64_t slab_stride(int P, int B) {
-  return slab_rz_len(P) + 8 * (int64_t)B;
-}
-
-__device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __restrict__ bm,
-                                           const double* __restrict__ v0,
-                                           const double* __restrict__ sd, const double* y,
-                                           double w, int nsym, double* __restrict__ rz,
-                                           double* __restrict__ ap, int P) {
-  SampleProj sp;
-  project(g, y, nsym > 2, sp);
-  for (int c = 0; c < nsym; ++c) {
-    double zs, phi, xr, yr, vlos;
-    copy_coords(sp, c, zs, phi, xr, yr, vlos);
-    const int cell = rzphi_cell(g, sp.R, zs * y[2], phi);
-    if (cell >= 0) rz[cell] += w;
-    const int a = aperture(g, bm, xr, yr);
-    if (a >= 0) {
-      double gw, H[5];
-      gh_terms(vlos, v0[a], sd[a], gw, H);
-      double* rec = ap + 8 * (int64_t)a;
-      const double gd = gw * w;
-      rec[0] += w;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) rec[1 + k] += gd * H[k];
-    }
-  }
-}
-
 __global__ __launch_bounds__(64) void integrate_library_kernel(
     smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
     const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
     const int32_t* __restrict__ bm, const double* __restrict__ v0, const double* __restrict__ sd,
     double* __restrict__ ws, double* __restrict__ valid_out, double* __restrict__ T_out,
-    double* __restrict__ tfin_out) {
+    double* __restrict__ tfin_out, double* __restrict__ ckpt, int ckpt_every) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n_int) return;
   const int P = g.nR * g.nz * g.nphi;
@@ -266,6 +107,13 @@ __global__ __launch_bounds__(64) void integrate_library_kernel(
   double t = 0.0, dt = dt0[i], T_acc = 0.0;
   const int nsym = g.symmetry ? 4 : 1;
   for (int step = 0; step < s.N_max; ++step) {
+    if (ckpt != nullptr && step % ckpt_every == 0) {
+      double* c = ckpt + ((int64_t)(step / ckpt_every) * n_int + i) * 8;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) c[j] = y[j];
+      c[6] = dt;
+      c[7] = t;
+    }
     const double used = bs3_attempt(p, s, t, y, k1, dt);
     if (used != 0.0) {
       bin_sample(g, bm, v0, sd, y, used, nsym, rz, ap, P);
@@ -441,13 +289,8 @@ __global__ __launch_bounds__(256) void density_target_kernel(
   __shared__ double red[4];
   const int64_t c = blockIdx.x;
   if (c >= P) return;
-  double acc = 0.0;
-  for (int64_t k = threadIdx.x; k < S; k += blockDim.x) {
-    const double R = Rg[c] + (smp[3 * k] - 0.5) * dR;
-    const double z = zg[c] + (smp[3 * k + 1] - 0.5) * dz;
-    const double ph = pg[c] + (smp[3 * k + 2] - 0.5) * dphi;
-    acc += rho_total(p, R * cos(ph), R * sin(ph), z) * R;
-  }
+  double acc = cell_mass_part<double>(p, fields_const<double>(p), Rg[c], zg[c], pg[c], dR, dz, dphi,
+                                     smp, S, threadIdx.x, blockDim.x);
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -524,15 +367,17 @@ static int check_grids(const smx_grids* g) {
   return 0;
 }
 
-int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
-                      const double* wb, const double* T_total, const double* dt0, int64_t n,
-                      int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
-                      const double* v0, const double* sdisp, void* workspace, double* lib,
-                      double* valid, double* T_int, double* t_final, void* stream) {
+static int build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                         const double* wb, const double* T_total, const double* dt0, int64_t n,
+                         int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
+                         const double* v0, const double* sdisp, void* workspace, double* lib,
+                         double* valid, double* T_int, double* t_final, double* ckpt,
+                         int32_t ckpt_every, void* stream) {
   SMX_REQUIRE(p && g && s && wb && T_total && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
                   workspace && lib && valid && T_int && t_final,
               "smx_build_library: null pointer");
   SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0, "smx_build_library: bad sizes");
+  SMX_REQUIRE(ckpt == nullptr || ckpt_every >= 1, "smx_build_library_ckpt: ckpt_every must be >= 1");
   if (int e = check_grids(g)) return e;
   if (n == 0) return 0;
   const int64_t n_int = n * r;
@@ -545,7 +390,7 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
   stage_begin(ST_INTEGRATE, st);
   hipLaunchKernelGGL(integrate_library_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p,
                      *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp, ws, valid, T_int,
-                     t_final);
+                     t_final, ckpt, ckpt_every);
   SMX_LAUNCHED();
   stage_end(ST_INTEGRATE, st, 1);
   const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
@@ -555,6 +400,26 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
   SMX_LAUNCHED();
   stage_end(ST_FINALIZE, st, 1);
   return 0;
+}
+
+int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                      const double* wb, const double* T_total, const double* dt0, int64_t n,
+                      int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
+                      const double* v0, const double* sdisp, void* workspace, double* lib,
+                      double* valid, double* T_int, double* t_final, void* stream) {
+  return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
+                       workspace, lib, valid, T_int, t_final, nullptr, 0, stream);
+}
+
+int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                           const double* wb, const double* T_total, const double* dt0, int64_t n,
+                           int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
+                           const double* v0, const double* sdisp, void* workspace, double* lib,
+                           double* valid, double* T_int, double* t_final, double* ckpt,
+                           int32_t ckpt_every, void* stream) {
+  SMX_REQUIRE(ckpt != nullptr, "smx_build_library_ckpt: null checkpoint buffer");
+  return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
+                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every, stream);
 }
 
 int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,

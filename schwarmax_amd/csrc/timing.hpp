@@ -7,7 +7,7 @@ namespace smx {
 
 enum Stage {
   ST_ICS = 0, ST_INTEGRATE, ST_FINALIZE, ST_GRAM, ST_FACTOR, ST_RHS, ST_ADMM, ST_MAPS,
-  ST_DENSITY, ST_ADMM_PRODUCT, ST_COUNT
+  ST_DENSITY, ST_ADMM_PRODUCT, ST_ADJOINT, ST_JACOBIAN, ST_COUNT
 };
 
 bool timing_on();

@@ -214,9 +214,13 @@ def fit_base_gaussians(dd, w_ics, rot, weights=None):
     dd['s'] = s
 
 
-def observe(dd, A_xy, A_h, mmpo, M_to_L, seed=1, noise=0.02, h_err=0.03):
+def observe(dd, A_xy, A_h, mmpo, M_to_L, seed=1, noise=0.02, h_err=0.03, floor=None):
     """Observed surface density and h1..h4 from library columns and Dirichlet
-    weights (n,), following the model's own conventions (model.py:217-272)."""
+    weights (n,), following the model's own conventions (model.py:217-272).
+
+    ``floor``: raise the surface density to at least floor x its median
+    positive value, so that no aperture has zero data (the likelihood divides
+    by 0.1 y_xy, likelihood.py:215, so zero data make its gradient NaN)."""
     rng = np.random.default_rng(seed)
     n = A_xy.shape[1]
     w = rng.dirichlet(np.ones(n)) * n
@@ -233,6 +237,9 @@ def observe(dd, A_xy, A_h, mmpo, M_to_L, seed=1, noise=0.02, h_err=0.03):
     B = sd_model.size
     sd = sd_model * mmpo / M_to_L
     dd['XY_density_data'] = sd * (1.0 + noise * rng.standard_normal(B))
+    if floor is not None:
+        pos = dd['XY_density_data'][dd['XY_density_data'] > 0]
+        dd['XY_density_data'] = np.maximum(dd['XY_density_data'], floor * np.median(pos))
     dd['XY_density_data_err'] = 0.01 * dd['XY_density_data'] + 1e-12
     for k in range(4):
         dd['h%d_data' % (k + 1)] = h[k] + noise * rng.standard_normal(B)

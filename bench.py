@@ -29,10 +29,10 @@ def log(*a):
     print('[bench]', *a, file=sys.stderr, flush=True)
 
 
-def make_workload(config, seed):
+def make_workload(config, seed, n=None):
     import torch
     from schwarmax_amd import synth, model as M, likelihood as L, potentials as POT
-    theta, dd, sched = synth.make_galaxy(config, seed=seed)
+    theta, dd, sched = synth.make_galaxy(config, seed=seed, n=n)
     ph, pb = L.theta_to_params(theta)
     s = M.schedule_of(sched)
     pot = POT.pack_potential(M.baryon_params(pb), ph)
@@ -126,6 +126,67 @@ def traffic_per_launch(config):
         return json.load(f).get('bytes_per_launch')
 
 
+def bench_batched(args):
+    """Batched evaluation: M thetas with uneven orbit counts in one call."""
+    import torch
+    from schwarmax_amd import _lib, synth, batch, likelihood as L
+    cfg = synth.CONFIGS[args.config]
+    lo, hi = cfg['n']
+    theta, dd, sched = make_workload(args.config, args.seed, n=hi)
+    rng = np.random.default_rng(args.seed + 1)
+    M = cfg['n_models']
+    thetas = np.stack(synth.batched_thetas(M, rng))
+    n_orbits = [int(v) for v in rng.integers(lo, hi + 1, size=M)]
+    B = int(dd['total_bins'])
+    P = int(dd['Rzphi_n_tot'])
+    T = cfg['T']
+
+    def step():
+        return batch.model_likelihood_batch(thetas, dd, B, P, n_orbits=n_orbits, schedule=sched,
+                                            nnls_maxiter=T, nnls_tol=args.nnls_tol)
+    for i in range(args.warmup):
+        t0 = time.time()
+        step()
+        torch.cuda.synchronize()
+        log(f'warmup {i}: {time.time() - t0:.1f}s')
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for i in range(args.steps):
+        v = step()
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    stages = _lib.timing_read()
+    _lib.timing_enable(False)
+    log(f'batch of {M}: {dt / args.steps:.2f} s; logL finite {int(torch.isfinite(v).sum())}/{M}')
+    out = {
+        'metric': 'batched model evals/s (64 theta, n ~ U[5k, 40k])', 'value': M * args.steps / dt,
+        'unit': 'model evals/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * dt / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic',
+        'config': {'workload': args.config, 'models': M, 'orbits_total': int(sum(n_orbits)),
+                   'orbits_min': min(n_orbits), 'orbits_max': max(n_orbits),
+                   'realizations': sched['n_realizations'], 'bs3_attempts': sched['N_max'],
+                   'apertures': B, 'rzphi_cells': P, 'bootstrap_rhs': dd['XY_standard_normal'].shape[0],
+                   'admm_iters': T},
+        'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
+    }
+    if args.compare:
+        t0 = time.time()
+        for m in range(M):
+            dm = dict(dd)
+            for k in ('w0', 'jeans_normals', 'jitter_uniforms'):
+                dm[k] = np.asarray(dd[k])[:n_orbits[m]]
+            L.model_likelihood(thetas[m], dm, B, P, schedule=sched, nnls_maxiter=T,
+                               nnls_tol=args.nnls_tol)
+            torch.cuda.synchronize()
+            log(f'plain model {m}: n={n_orbits[m]} {time.time() - t0:.1f}s')
+        seq = time.time() - t0
+        out['plain_sequential_evals_per_s'] = M / seq
+        out['batched_over_plain'] = (M * args.steps / dt) / (M / seq)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -135,6 +196,8 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--nnls-tol', type=float, default=0.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--compare', action='store_true',
+                    help='batched configs: also time the same models one by one (plain path)')
     ap.add_argument('--calibrate-copy', type=int, default=0,
                     help='only run a device copy of this many MiB (HBM counter calibration)')
     args = ap.parse_args()
@@ -158,6 +221,8 @@ def main():
     from schwarmax_amd import _lib, likelihood as L
     from oracle import opcount
     from oracle import model as OM
+    if args.config.startswith('batched'):
+        return bench_batched(args)
     theta, dd, sched = make_workload(args.config, args.seed)
     B = int(dd['total_bins'])
     P = int(dd['Rzphi_n_tot'])

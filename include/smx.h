@@ -184,6 +184,30 @@ int smx_density_target_jac(const smx_potential* p, const double* R_grid, const d
                            const double* phi_grid, int64_t P, double dR, double dz, double dphi,
                            const double* sample, int64_t S, double* jac, void* stream);
 
+/* ---- batched evaluation (many theta, one launch per stage) -------------- */
+/* Workspace for smx_build_library_batch: the persistent grid's slab pool plus
+ * the model and work-item tables (total_items = sum over models of
+ * ceil(n_orbits / (64 / r))). */
+int64_t smx_batch_workspace_bytes(const smx_grids* g, int32_t n_models, int64_t total_items);
+/* ICs and orbit libraries of n_models models in two launches (the reference
+ * evaluates model.py:104-258 per theta).  Model m has its own potential,
+ * grids (the sky rotation differs; the binning geometry must be shared),
+ * schedule, N_step_per_orb / N_dynamical_time (model.py:163-169) and orbit
+ * count n_orbits[m] <= the stars in w0 (n, 3), normals (n, 3) and uniforms
+ * (n, r_total, 6), of which it uses the first n_orbits[m].  The host arrays
+ * wb, T_total, dt0, lib hold per-model DEVICE pointers ((n_m, r, 6), (n_m, r),
+ * (n_m, r), (n_m, P + 6B)).  Integration is one persistent launch over work
+ * items of 64/r columns taken from a device queue, longest schedules first. */
+int smx_build_library_batch(int32_t n_models, const smx_potential* pots, const smx_grids* grids,
+                            const smx_schedule* scheds, const double* w0, const double* normals,
+                            const double* uniforms, const int64_t* n_orbits, int32_t r,
+                            int32_t r_total, const double* N_step_per_orb,
+                            const double* N_dynamical_time, const int32_t* bin_mapping,
+                            const double* num_per_bin, const double* v0, const double* sdisp,
+                            double* const* wb, double* const* T_total, double* const* dt0,
+                            double* const* lib, void* workspace, int64_t workspace_bytes,
+                            void* stream);
+
 /* ---- NNLS (nnls.py:6-166) ----------------------------------------------- */
 /* Bytes of NNLS workspace for n orbits, m rows, K right-hand sides. */
 int64_t smx_nnls_workspace_bytes(int64_t n, int64_t m, int64_t K);

@@ -80,6 +80,12 @@ _SIGS = [
     ('smx_density_target_jac', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double,
       ctypes.c_double, _P, _I64, _P, _P]),
+    ('smx_batch_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I32, _I64]),
+    ('smx_build_library_batch', ctypes.c_int,
+     [_I32, ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _P, ctypes.POINTER(_I64), _I32, _I32, ctypes.POINTER(ctypes.c_double),
+      ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, ctypes.POINTER(_P), ctypes.POINTER(_P),
+      ctypes.POINTER(_P), ctypes.POINTER(_P), _P, _I64, _P]),
     ('smx_build_library_traj', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

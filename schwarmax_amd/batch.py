@@ -1,0 +1,147 @@
+"""Batched model evaluation: many parameter vectors theta, one galaxy.
+
+The reference evaluates model_likelihood (likelihood.py:96-244) once per theta
+inside its MCMC driver.  Here a batch of M thetas, each with its own orbit
+count n_m (the first n_m orbits of dict_data), runs as:
+
+* the Jeans ICs of all (model, star) pairs in one launch and the orbit
+  libraries of all models in one persistent launch fed by a device work queue
+  (``smx_build_library_batch``), on a side stream, group by group under a
+  library-memory budget;
+* per model, on the main stream, the density target, bootstrap NNLS (all K
+  right-hand sides batched) and the chi^2, overlapping the next group's
+  integration: the integrator is FP64-VALU bound, the NNLS MFMA/HBM bound.
+
+Every model's value equals model_likelihood(theta_m, ...) on the first n_m
+orbits.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import potentials as POT
+from . import model as MD
+from .likelihood import theta_to_params, grid_args, bootstrap_loglike
+
+
+def _groups(n_orbits, row_bytes, budget):
+    groups, cur, size = [], [], 0
+    for m, n in enumerate(n_orbits):
+        b = int(n) * row_bytes
+        if cur and size + b > budget:
+            groups.append(cur)
+            cur, size = [], 0
+        cur.append(m)
+        size += b
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
+                           sigma_amplify=1.0, schedule=None, nnls_maxiter=200, nnls_tol=0.0,
+                           library_budget_bytes=48 << 30, return_aux=False):
+    """logL for each row of thetas (M, 12); returns a CUDA tensor (M,).
+
+    n_orbits: per-model orbit counts (default: all orbits for every model).
+    schedule: one schedule dict, or a list of M (their N_max may differ).
+    """
+    thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
+    M = thetas.shape[0]
+    L = _lib.lib()
+    w0_all = np.asarray(dict_data['w0'])[:, :3]
+    n_all = w0_all.shape[0]
+    n_orbits = [n_all] * M if n_orbits is None else [int(v) for v in n_orbits]
+    if len(n_orbits) != M or min(n_orbits) < 1 or max(n_orbits) > n_all:
+        raise ValueError("n_orbits must hold M counts in [1, number of orbits in dict_data]")
+    scheds = schedule if isinstance(schedule, (list, tuple)) else [schedule] * M
+    scheds = [MD.schedule_of(s) for s in scheds]
+    r = int(scheds[0]['n_realizations'])
+    if any(int(s['n_realizations']) != r for s in scheds):
+        raise ValueError("all models must use the same number of realizations")
+    Rl, Rn, xyl, xyn = grid_args(dict_data)
+    pots, grids, sstructs, pbs = [], [], [], []
+    for m in range(M):
+        ph, pb = theta_to_params(thetas[m])
+        pbs.append(pb)
+        pots.append(POT.pack_potential(MD.baryon_params(pb), ph))
+        rot = MD.makeRotationMatrix(pb['alpha'], pb['beta'], pb['gamma'])
+        grids.append(MD.make_grids(num_Vbin, Rn, Rl, xyl, xyn, rot, symmetry=True))
+        sstructs.append(MD.make_schedule(scheds[m], -float(pb['Omega_bar'])))
+    if grids[0].nR * grids[0].nz * grids[0].nphi != int(Rzphi_n_tot):
+        raise ValueError("Rzphi_n_tot must equal prod(Rzphi_n_grid)")
+    P, B = grids[0].nR * grids[0].nz * grids[0].nphi, grids[0].num_vbin
+    rows = P + 6 * B
+    dev = _lib.dev
+    w0 = dev(w0_all)
+    normals = dev(dict_data['jeans_normals'])
+    uni = dev(dict_data['jitter_uniforms'])
+    r_total = uni.shape[1]
+    bm = dev(dict_data['bin_mapping'], dtype=torch.int32)
+    npb, v0, sd = dev(dict_data['num_per_bin']), dev(dict_data['v0']), dev(dict_data['s'])
+    if bm.numel() < grids[0].nX * grids[0].nY:
+        raise ValueError("bin_mapping must have nX*nY entries (plus the sentinel)")
+
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    side.wait_stream(main)
+    groups = _groups(n_orbits, rows * 8, library_budget_bytes)
+    cols = 64 // r
+    ws = None
+
+    def launch(g):
+        nonlocal ws
+        k = len(g)
+        items = sum((n_orbits[m] + cols - 1) // cols for m in g)
+        need = int(L.smx_batch_workspace_bytes(grids[g[0]], k, items))
+        with torch.cuda.stream(side):
+            if ws is None or ws.numel() < need:
+                ws = torch.empty(need, dtype=torch.uint8, device='cuda')
+            out = []
+            for m in g:
+                n = n_orbits[m]
+                out.append(dict(wb=torch.empty((n, r, 6), dtype=torch.float64, device='cuda'),
+                                Tt=torch.empty((n, r), dtype=torch.float64, device='cuda'),
+                                dt0=torch.empty((n, r), dtype=torch.float64, device='cuda'),
+                                lib=torch.empty((n, rows), dtype=torch.float64, device='cuda')))
+            arr = lambda T, vals: (T * k)(*vals)  # noqa: E731
+            P_ = ctypes.c_void_p
+            _lib.check(L.smx_build_library_batch(
+                k, arr(_lib.SmxPotential, [pots[m] for m in g]),
+                arr(_lib.SmxGrids, [grids[m] for m in g]),
+                arr(_lib.SmxSchedule, [sstructs[m] for m in g]),
+                _lib.ptr(w0), _lib.ptr(normals), _lib.ptr(uni),
+                arr(ctypes.c_int64, [n_orbits[m] for m in g]), r, r_total,
+                arr(ctypes.c_double, [float(scheds[m]['N_step_per_orb']) for m in g]),
+                arr(ctypes.c_double, [float(scheds[m]['N_dynamical_time']) for m in g]),
+                _lib.ptr(bm), _lib.ptr(npb), _lib.ptr(v0), _lib.ptr(sd),
+                arr(P_, [o['wb'].data_ptr() for o in out]),
+                arr(P_, [o['Tt'].data_ptr() for o in out]),
+                arr(P_, [o['dt0'].data_ptr() for o in out]),
+                arr(P_, [o['lib'].data_ptr() for o in out]),
+                _lib.ptr(ws), ws.numel(), _lib.stream_ptr()), 'smx_build_library_batch')
+            ev = torch.cuda.Event()
+            ev.record(side)
+        for o in out:
+            for t in o.values():
+                t.record_stream(main)
+        return out, ev
+
+    logl = torch.empty(M, dtype=torch.float64, device='cuda')
+    aux = {}
+    pending = launch(groups[0])
+    for gi, g in enumerate(groups):
+        out, ev = pending
+        if gi + 1 < len(groups):
+            pending = launch(groups[gi + 1])        # integrates while this group's NNLS runs
+        main.wait_event(ev)
+        for m, o in zip(g, out):
+            md, info = MD.model_from_library(o['lib'], P, B, pbs[m], dict_data, nnls_maxiter, 1.0,
+                                             (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0)), nnls_tol)
+            logl[m] = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
+            if return_aux:
+                aux[m] = dict(lib=o['lib'], weights=md['weights'], nnls_iters=info['nnls_iters'])
+        del out
+    return (logl, aux) if return_aux else logl

@@ -169,12 +169,10 @@ def compute_model_single(W, lib, P, B, gamma_kin, v0, s):
 def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
              Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, nnls_maxiter,
              regularization, schedule, traj, channel_w, nnls_tol):
-    from .nnls import solve_nnls_admm_bootstrap_lib
     _canonical(density_func, potential_func)
     sched = schedule_of(schedule)
     pb = baryon_params(params_disk_rho)
     pot = POT.pack_potential(pb, params_halo_pot)
-    lm = 1.0 / float(params_disk_rho['mass_to_light_ratio'])
     Omega_bar = float(params_disk_rho['Omega_bar'])
     rot = makeRotationMatrix(float(params_disk_rho['alpha']), float(params_disk_rho['beta']),
                              float(params_disk_rho['gamma']))
@@ -185,9 +183,19 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
         raise ValueError("Rzphi_n_tot must equal prod(Rzphi_n_grid)")
     wb, Tt, dt0 = initial_conditions(pot, dict_data, r, sched)
     libd = build_library(pot, grids, make_schedule(sched, -Omega_bar), wb, Tt, dt0, dict_data, traj)
-    lib, P, B = libd['lib'], libd['P'], libd['B']
-    n = lib.shape[0]
+    out, aux = model_from_library(libd['lib'], libd['P'], libd['B'], params_disk_rho, dict_data,
+                                  nnls_maxiter, regularization, channel_w, nnls_tol)
+    aux.update(libd=libd, rot=rot, Omega_bar=Omega_bar, wb=wb, T_total=Tt, dt0=dt0)
+    return out, aux
 
+
+def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regularization,
+                       channel_w, nnls_tol=0.0):
+    """Everything build_model does after the library (model.py:192-286):
+    density target, data normalisation, bootstrap NNLS and the model maps."""
+    from .nnls import solve_nnls_admm_bootstrap_lib
+    lm = 1.0 / float(params_disk_rho['mass_to_light_ratio'])
+    n = lib.shape[0]
     y_R = density_target(params_disk_rho, dict_data)
     sig_R = 0.02 * y_R + 1e-10
     D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
@@ -220,9 +228,7 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     d2 = d2 * mmpo * lm
     out = {'density_3d': y_R, 'density_3d_model': d3, 'surface_density': d2, 'h1': h1, 'h2': h2,
            'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
-    aux = dict(libd=libd, mmpo=mmpo, rot=rot, Omega_bar=Omega_bar, n=n, nnls_iters=info['iters'],
-               wb=wb, T_total=Tt, dt0=dt0)
-    return out, aux
+    return out, dict(mmpo=mmpo, n=n, nnls_iters=info['iters'])
 
 
 def build_model(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,

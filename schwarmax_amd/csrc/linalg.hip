@@ -637,73 +637,99 @@ __global__ void transpose_into_kernel(const T* __restrict__ src, int64_t lds, in
 }
 
 // ---------------------------------------------------------------------------
-// SPD inverse in FP64 by recursive Schur complements, in place on the full
-// matrix:
+// SPD inverse in FP64 by recursive Schur complements, in place:
 //   [A11 A21^T; A21 A22]^-1 with X = A11^-1, T = A21 X, S = A22 - T A21^T:
-//   A22 <- S^-1,  A21 <- -S^-1 T,  A11 <- X - T^T A21,  A12 <- A21^T.
-// 4n^3/3 flops in FP64 MFMA GEMMs; 64 x 64 base blocks by Gauss-Jordan in
-// registers (SPD: no pivoting).  The Schur complements must stay positive
-// definite: in FP32 their error, ~ eps cond(A11) |A21|^2, destroys that at
-// the library's conditioning, in FP64 it is ~1e-9 of the smallest eigenvalue.
+//   A22 <- S^-1,  A21 <- -S^-1 T,  A11 <- X - T^T A21.
+// The recursion reads and writes the lower triangle: the two symmetric
+// updates (S and the new A11) are formed on their lower triangles only, and
+// X and S^-1 are mirrored before they serve as full GEMM operands; the caller
+// mirrors the final lower triangle.  n^3 flops in FP64 MFMA GEMMs (4n^3/3
+// with full updates); 64 x 64 base blocks by Gauss-Jordan in one wavefront
+// (SPD: no pivoting).  The Schur complements must stay positive definite: in
+// FP32 their error, ~ eps cond(A11) |A21|^2, destroys that at the library's
+// conditioning, in FP64 it is ~1e-9 of the smallest eigenvalue.
 // ---------------------------------------------------------------------------
 constexpr int INV_BASE = 64;
 
-__global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
-  // thread t owns column j = t & 63 for rows h*16 .. h*16+15, h = t >> 6
+// One wavefront, symmetric sweep operator: lane j holds column j of the
+// (nb <= 64) block in registers, read from the lower triangle and padded with
+// the identity.  Sweeping pivot k keeps the matrix symmetric, so the pivot
+// column is the pivot row, which every lane publishes with one LDS store:
+//   a_kk <- -1/a_kk,  a_ik <- a_ik/a_kk,  a_ij <- a_ij - a_ik a_kj / a_kk.
+// Sweeping all pivots leaves -A^-1; the full block is written back negated.
+// The column registers rotate by one per pivot (v[r] holds row (k + r) mod
+// 64 at step k), so every register index is static.
+__global__ __launch_bounds__(64) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
   __shared__ double pr[INV_BASE];
-  __shared__ double pc[INV_BASE];
-  const int j = threadIdx.x & (INV_BASE - 1), h = threadIdx.x >> 6;
-  double v[16];
+  const int j = threadIdx.x;
+  double v[INV_BASE];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int i = h * 16 + r;
-    v[r] = (i < nb && j < nb) ? A[(int64_t)i * lda + j] : ((i == j) ? 1.0 : 0.0);
+  for (int i = 0; i < INV_BASE; ++i) {
+    double x = (i == j) ? 1.0 : 0.0;
+    if (i < nb && j < nb) x = (i >= j) ? A[(int64_t)i * lda + j] : A[(int64_t)j * lda + i];
+    v[i] = x;
   }
-  for (int k = 0; k < nb; ++k) {
-    if (h == (k >> 4)) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        if (r == (k & 15)) pr[j] = v[r];
-    }
-    if (j == k) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) pc[h * 16 + r] = v[r];
-    }
+  for (int k = 0; k < INV_BASE; ++k) {
+    pr[j] = v[0];                       // a_kj = a_jk
     __syncthreads();
     const double p = 1.0 / pr[k];
-    const double prj = pr[j];
+    const double akj = v[0];
+    const bool pivot_col = (j == k);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = h * 16 + r;
-      double x;
-      if (i == k) x = (j == k) ? p : prj * p;
-      else if (j == k) x = -pc[i] * p;
-      else x = v[r] - pc[i] * prj * p;
-      v[r] = x;
+    for (int r = 1; r < INV_BASE; ++r) {
+      const double aik = pr[(k + r) & (INV_BASE - 1)];
+      v[r] = pivot_col ? aik * p : v[r] - aik * akj * p;
     }
+    const double vk = pivot_col ? -p : akj * p;
+#pragma unroll
+    for (int r = 0; r < INV_BASE - 1; ++r) v[r] = v[r + 1];
+    v[INV_BASE - 1] = vk;
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int i = h * 16 + r;
-    if (i < nb && j < nb) A[(int64_t)i * lda + j] = v[r];
+  for (int i = 0; i < INV_BASE; ++i)
+    if (i < nb && j < nb) A[(int64_t)i * lda + j] = -v[i];
+}
+
+// Upper triangle of the n x n matrix <- its lower triangle, through 32 x 32
+// LDS tiles (coalesced on both sides).
+template <class T>
+__global__ __launch_bounds__(256) void mirror_lower_kernel(T* __restrict__ A, int64_t n,
+                                                           int64_t lda, int nt) {
+  __shared__ T tile[32][33];
+  // block b -> destination tile (bi, bj), bj >= bi, row-major over the upper triangle
+  const int64_t b = blockIdx.x;
+  int bi = (int)(((2.0 * nt + 1.0) - sqrt((2.0 * nt + 1.0) * (2.0 * nt + 1.0) - 8.0 * b)) * 0.5);
+  while (bi > 0 && (int64_t)bi * (2 * nt - bi + 1) / 2 > b) --bi;
+  while ((int64_t)(bi + 1) * (2 * nt - bi) / 2 <= b) ++bi;
+  const int bj = (int)(b - (int64_t)bi * (2 * nt - bi + 1) / 2 + bi);
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // source: tile (bj, bi) of the lower triangle
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t gi = (int64_t)bj * 32 + r, gj = (int64_t)bi * 32 + tx;
+    if (gi < n && gj < n) tile[r][tx] = A[gi * lda + gj];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t gi = (int64_t)bi * 32 + r, gj = (int64_t)bj * 32 + tx;
+    if (gi < n && gj < n && gj > gi) A[gi * lda + gj] = tile[tx][r];
   }
 }
 
 template <class T>
-__global__ void mirror_lower_kernel(T* __restrict__ A, int64_t n, int64_t lda) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int64_t i = idx / n, j = idx - i * n;
-  if (j > i) A[i * lda + j] = A[j * lda + i];
+static void mirror_lower(hipStream_t st, T* A, int64_t n, int64_t lda) {
+  const int nt = (int)((n + 31) / 32);
+  const unsigned tiles = (unsigned)((int64_t)nt * (nt + 1) / 2);
+  hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(tiles), dim3(256), 0, st, A, n, lda, nt);
 }
 
-// Every product is in NT form (both operands k-contiguous):
-//   Tt = X A21^T,  T = Tt^T,  A22 -= T A21^T,  A21 = -S^-1 Tt^T,  A12 = A21^T,
-//   A11 -= Tt A12^T.  work holds 2 n1 n2 per level, n^2 in total suffices.
+// Products in NT form (both operands k-contiguous):
+//   Tt = X A21^T,  T = Tt^T,  A22 -= T A21^T (lower),  A21 = -S^-1 Tt^T,
+//   A12 = A21^T,  A11 -= Tt A12^T (lower).  work holds 2 n1 n2 per level, n^2
+// in total suffices.
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
-    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
+    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(64), 0, st, A, lda, (int)n);
     SMX_LAUNCHED();
     return 0;
   }
@@ -717,17 +743,27 @@ static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double
   double* Tm = work + n1 * n2;            // n2 x n1
   double* deeper = work + 2 * n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
+  mirror_lower(st, A11, n1, lda);
   if (int e = gemm_nt<double>(st, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, Tt, n2)) return e;
   hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
                      Tt, n2, n1, n2, Tm, n1);
   SMX_LAUNCHED();
-  if (int e = gemm_nt<double>(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda)) return e;
+  if (int e = gemm_nt<double>(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda, 1)) return e;
   if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
+  mirror_lower(st, A22, n2, lda);
   if (int e = gemm_nt<double>(st, n2, n1, n2, -1.0, A22, lda, Tt, n2, 0.0, A21, lda)) return e;
   hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
                      A21, lda, n2, n1, A12, lda);
   SMX_LAUNCHED();
-  return gemm_nt<double>(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda);
+  return gemm_nt<double>(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda, 1);
+}
+
+// Full inverse of the SPD matrix whose lower triangle is given.
+static int spd_inverse64(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
+  if (int e = inv_spd_rec(st, A, n, lda, work)) return e;
+  mirror_lower(st, A, n, lda);
+  SMX_LAUNCHED();
+  return 0;
 }
 
 // Q (FP32, full) -> FP64 with Q += reg I, rho = tr(Q)/n, Q += rho I (nnls.py:115-121)
@@ -788,7 +824,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   SMX_LAUNCHED();
   // Q = U^T U: lower half on MFMA (FP32), mirrored, widened to FP64
   if (int e = gemm_nt<float>(st, n, n, ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1)) return e;
-  hipLaunchKernelGGL(mirror_lower_kernel<float>, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n, n);
+  mirror_lower<float>(st, w.Q, n, n);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
   SMX_LAUNCHED();
@@ -798,7 +834,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   stage_end(ST_GRAM, st, 5);
   // Q + rho I <- (Q + rho I)^-1 in place, FP64
   stage_begin(ST_FACTOR, st);
-  if (int e = inv_spd_rec(st, w.Q64, n, n, w.work64)) return e;
+  if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
   stage_end(ST_FACTOR, st, 1);
   stage_begin(ST_RHS, st);
   // C = -Y U
@@ -892,7 +928,7 @@ int smx_gemm_nt_f64(int64_t M, int64_t N, int64_t K, double alpha, const double*
 
 int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream) {
   SMX_REQUIRE(A && work && n > 0 && n < ((int64_t)1 << 31), "smx_spd_inverse_f64: bad arguments");  // work: n*n + n doubles
-  return inv_spd_rec(as_stream(stream), A, n, n, work);
+  return spd_inverse64(as_stream(stream), A, n, n, work);
 }
 
 int smx_model_maps(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const double* W,

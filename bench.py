@@ -126,6 +126,46 @@ def traffic_per_launch(config):
         return json.load(f).get('bytes_per_launch')
 
 
+def bench_jackknife(args):
+    """calculate_delta_chi2 (likelihood.py:248-335): one group per aperture."""
+    import torch
+    from schwarmax_amd import _lib, jackknife as J
+    theta, dd, sched = make_workload(args.config, args.seed)
+    B = int(dd['total_bins'])
+    P = int(dd['Rzphi_n_tot'])
+
+    def step():
+        return J.calculate_delta_chi2(theta, dd, B, P, schedule=sched, nnls_maxiter=200,
+                                      return_aux=True)
+    for i in range(args.warmup):
+        t0 = time.time()
+        step()
+        torch.cuda.synchronize()
+        log(f'warmup {i}: {time.time() - t0:.1f}s')
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for i in range(args.steps):
+        v, aux = step()
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    stages = _lib.timing_read()
+    _lib.timing_enable(False)
+    log(f'delta chi2 {float(v):.6g}; {dt / args.steps:.2f} s; Neumann terms {aux["neumann_terms"]}')
+    out = {
+        'metric': 'jackknife delta-chi2 evals/s (one group per aperture, 1000 replicates)',
+        'value': args.steps / dt, 'unit': 'evals/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': 1e3 * dt / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic',
+        'config': {'workload': args.config, 'orbits': int(dd['w0'].shape[0]), 'replicates': B,
+                   'apertures': B, 'rzphi_cells': P, 'admm_iters': 200,
+                   'neumann_terms': aux['neumann_terms'],
+                   'max_shift_over_rho': float(aux['delta'].max()) / aux['rho']},
+        'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
+    }
+    print(json.dumps(out))
+
+
 def bench_batched(args):
     """Batched evaluation: M thetas with uneven orbit counts in one call."""
     import torch
@@ -223,6 +263,8 @@ def main():
     from oracle import model as OM
     if args.config.startswith('batched'):
         return bench_batched(args)
+    if args.config.startswith('jackknife'):
+        return bench_jackknife(args)
     theta, dd, sched = make_workload(args.config, args.seed)
     B = int(dd['total_bins'])
     P = int(dd['Rzphi_n_tot'])

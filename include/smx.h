@@ -184,6 +184,22 @@ int smx_density_target_jac(const smx_potential* p, const double* R_grid, const d
                            const double* phi_grid, int64_t P, double dR, double dz, double dphi,
                            const double* sample, int64_t S, double* jac, void* stream);
 
+/* ---- NNLS operator (the jackknife's shared solve) ----------------------- */
+/* The ADMM operator alone, for many solves that share one Gram up to
+ * low-rank changes (model.py:490-577 re-solves the NNLS per jackknife
+ * replicate): M = (U^T U + (lambda/n) I + rho I)^-1, rho = tr(.)/n
+ * (nnls.py:115-121), formed once in `workspace` (sized by
+ * smx_nnls_operator_bytes for at most K_max right-hand sides; U as in
+ * smx_nnls_admm).  rho_out: one device double. */
+int64_t smx_nnls_operator_bytes(int64_t n, int64_t m, int64_t K_max);
+int smx_nnls_operator(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
+                      const double* row_scale, double lambda_reg, int64_t K_max, void* workspace,
+                      double* rho_out, void* stream);
+/* out (K, n) = R (K, n) M for K <= K_max rows, FP32 in and out (the ADMM
+ * product: fp16 hi/lo splits, three MFMAs, FP32 accumulation). */
+int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
+                            int64_t K, float* out, void* stream);
+
 /* ---- batched evaluation (many theta, one launch per stage) -------------- */
 /* Workspace for smx_build_library_batch: the persistent grid's slab pool plus
  * the model and work-item tables (total_items = sum over models of

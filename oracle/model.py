@@ -119,24 +119,9 @@ def _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_li
     return lib
 
 
-def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
-             Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid,
-             nnls_maxiter, regularization, schedule, traj, channel_w, tape=None, W_fixed=None):
-    if density_func is not P.density_func or potential_func is not P.potential_func:
-        raise NotImplementedError("only the canonical density/potential are restated")
-    sched = schedule_of(schedule)
-    pb = baryon_params(params_disk_rho)
-    ph = params_halo_pot
+def _targets(dict_data, params_disk_rho, n):
+    """Density target and normalised data (model.py:192-258)."""
     lm = 1 / params_disk_rho['mass_to_light_ratio']
-    Omega_bar = params_disk_rho['Omega_bar']
-    rot = make_rotation_matrix(params_disk_rho['alpha'], params_disk_rho['beta'], params_disk_rho['gamma'])
-    lib = _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_lim_grid,
-                   xy_lim_grid, xy_n_grid, sched, traj, tape)
-    A_Rzphi = lib['Rzphi'].T
-    A_xy = lib['surface_density'].T
-    A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
-    n = A_Rzphi.shape[1]
-
     y_Rzphi = density_target(dict_data, params_disk_rho)
     sig_Rzphi = 0.02 * y_Rzphi + 1e-10
     dd = {k: np.asarray(v, dtype=np.float64) for k, v in dict_data.items()
@@ -160,7 +145,33 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     M_per_bin = y_xy * num_per_bin * area
     norm_GH = 1.0 + y_h[3] * np.sqrt(6.0) / 4.0
     gamma_kin = M_per_bin / (norm_GH * 2.0 * np.sqrt(np.pi) * s_ap + EPSILON)
+    return dict(lm=lm, y_Rzphi=y_Rzphi, sig_Rzphi=sig_Rzphi, y_xy=y_xy, sig_xy=sig_xy, y_h=y_h,
+                sig_h=sig_h, mmpo=mmpo, gamma_kin=gamma_kin, s_ap=s_ap, v0_ap=v0_ap, dd=dd)
 
+
+def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
+             Rzphi_n_tot, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid,
+             nnls_maxiter, regularization, schedule, traj, channel_w, tape=None, W_fixed=None):
+    if density_func is not P.density_func or potential_func is not P.potential_func:
+        raise NotImplementedError("only the canonical density/potential are restated")
+    sched = schedule_of(schedule)
+    pb = baryon_params(params_disk_rho)
+    ph = params_halo_pot
+    lm = 1 / params_disk_rho['mass_to_light_ratio']
+    Omega_bar = params_disk_rho['Omega_bar']
+    rot = make_rotation_matrix(params_disk_rho['alpha'], params_disk_rho['beta'], params_disk_rho['gamma'])
+    lib = _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_lim_grid,
+                   xy_lim_grid, xy_n_grid, sched, traj, tape)
+    A_Rzphi = lib['Rzphi'].T
+    A_xy = lib['surface_density'].T
+    A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
+    n = A_Rzphi.shape[1]
+
+    t = _targets(dict_data, params_disk_rho, n)
+    y_Rzphi, sig_Rzphi, y_xy, sig_xy = t['y_Rzphi'], t['sig_Rzphi'], t['y_xy'], t['sig_xy']
+    y_h, sig_h, mmpo, gamma_kin = t['y_h'], t['sig_h'], t['mmpo'], t['gamma_kin']
+    v0_ap, s_ap = t['v0_ap'], t['s_ap']
+    dd = t['dd']
     y_xy_boot = y_xy[None, :] + dd['XY_standard_normal'] * sig_xy[None, :]
     y_h_boot = [y_h[k][None, :] + dd['h%d_standard_normal' % (k + 1)] * sig_h[k][None, :]
                 for k in range(4)]
@@ -220,6 +231,52 @@ def get_model_with_orbit(density_func, potential_func, params_halo_pot, params_d
     return res
 
 
+def model_deltaChi2_jackknife(density_func, potential_func, chi2_func, params_halo_pot,
+                              params_disk_rho, dict_data, num_Vbin, Rzphi_n_tot=360,
+                              Rzphi_n_grid=(10, 6, 6),
+                              Rzphi_lim_grid=((0, 10.), (-3, 3), (-np.pi, np.pi)),
+                              xy_lim_grid=((-10., 10.), (-3., 3.)), xy_n_grid=(60, 40),
+                              nnls_maxiter=200, regularization=1.0, n_groups=100, batch_size=50,
+                              schedule=None, return_all=False):
+    """Delete-d jackknife (model.py:290-603): replicate g raises the sigma of
+    the bins b with b % n_groups == g to 1e20 and re-solves the single-RHS
+    NNLS with its own Gram and Cholesky; chi2_func per replicate; std over
+    replicates.  batch_size only bounds the reference's memory."""
+    if density_func is not P.density_func or potential_func is not P.potential_func:
+        raise NotImplementedError("only the canonical density/potential are restated")
+    sched = schedule_of(schedule)
+    pb = baryon_params(params_disk_rho)
+    rot = make_rotation_matrix(params_disk_rho['alpha'], params_disk_rho['beta'],
+                               params_disk_rho['gamma'])
+    lib = _library(dict_data, pb, params_halo_pot, params_disk_rho['Omega_bar'], rot, num_Vbin,
+                   Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, sched, False)
+    A_Rzphi = lib['Rzphi'].T
+    A_xy = lib['surface_density'].T
+    A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
+    n = A_Rzphi.shape[1]
+    t = _targets(dict_data, params_disk_rho, n)
+    lum = t['mmpo'] * t['lm']
+    B = t['sig_xy'].shape[0]
+    BIG = 1e20
+    chi2 = []
+    for g in range(n_groups):
+        dropped = (np.arange(B) % n_groups) == g
+        sxy = np.where(dropped, BIG, t['sig_xy'])
+        sh = [np.where(dropped, BIG, sk) for sk in t['sig_h']]
+        w = NN.solve_nnls_admm(A_Rzphi, A_xy, A_h, t['y_Rzphi'], t['y_xy'], t['gamma_kin'],
+                               t['y_h'], t['sig_Rzphi'], sxy, sh, lambda_reg=regularization,
+                               maxiter=nnls_maxiter, w_rzphi=np.sqrt(5.0), w_xy=np.sqrt(5.0),
+                               w_h=np.sqrt(1.0))
+        d3, d2, h1, h2, h3, h4, _, _ = compute_model_single(w, A_Rzphi, A_xy, A_h, t['gamma_kin'],
+                                                            t['v0_ap'], t['s_ap'])
+        chi2.append(chi2_func(t['y_Rzphi'], d3, t['sig_Rzphi'], t['y_xy'] * lum, d2 * lum,
+                              sxy * lum, t['y_h'][0], h1, sh[0], t['y_h'][1], h2, sh[1],
+                              t['y_h'][2], h3, sh[2], t['y_h'][3], h4, sh[3]))
+    chi2 = np.asarray(chi2)
+    out = np.std(chi2)
+    return (out, chi2) if return_all else out
+
+
 # --------------------------------------------------------------------------
 # likelihood.py
 # --------------------------------------------------------------------------
@@ -274,6 +331,28 @@ def grid_args(dict_data):
                          dtype=np.float64)
     xy_lim = np.array([dict_data['X_minmax'], dict_data['Y_minmax']], dtype=np.float64)
     return Rzphi_lim, np.asarray(dict_data['Rzphi_n_grid']), xy_lim, np.asarray(dict_data['nX_nY'])
+
+
+def chi2_func(y_Rzphi, y_Rzphi_model, sigma_Rzphi, y_xy, y_xy_model, sigma_xy,
+              y_h1, y_h1_model, sigma_h1, y_h2, y_h2_model, sigma_h2,
+              y_h3, y_h3_model, sigma_h3, y_h4, y_h4_model, sigma_h4):
+    """likelihood.py:73-94 (the density terms are formed but, as there, not
+    returned)."""
+    def res(y, ym, sg):
+        return np.nansum(((y - ym) / sg) ** 2)
+    return -0.5 * (res(y_h1, y_h1_model, sigma_h1) + res(y_h2, y_h2_model, sigma_h2) +
+                   res(y_h3, y_h3_model, sigma_h3) + res(y_h4, y_h4_model, sigma_h4))
+
+
+def calculate_delta_chi2(params, dict_data, num_Vbin, Rzphi_n_tot, schedule=None,
+                         nnls_maxiter=200):
+    """likelihood.py:248-335: jackknife with one group per aperture."""
+    ph, pb = theta_to_params(params)
+    Rzphi_lim, Rzphi_n, xy_lim, xy_n = grid_args(dict_data)
+    return model_deltaChi2_jackknife(P.density_func, P.potential_func, chi2_func, ph, pb,
+                                     dict_data, num_Vbin, Rzphi_n_tot, Rzphi_n, Rzphi_lim, xy_lim,
+                                     xy_n, nnls_maxiter=nnls_maxiter, regularization=1.0,
+                                     n_groups=num_Vbin, batch_size=25, schedule=schedule)
 
 
 def bootstrap_loglike(model_dict, dict_data, sigma_amplify=1.0):

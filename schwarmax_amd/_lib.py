@@ -80,6 +80,9 @@ _SIGS = [
     ('smx_density_target_jac', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, ctypes.c_double, ctypes.c_double,
       ctypes.c_double, _P, _I64, _P, _P]),
+    ('smx_nnls_operator_bytes', _I64, [_I64, _I64, _I64]),
+    ('smx_nnls_operator', ctypes.c_int, [_P, _I64, _I64, _I64, _P, ctypes.c_double, _I64, _P, _P, _P]),
+    ('smx_nnls_operator_apply', ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, _P, _P]),
     ('smx_batch_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I32, _I64]),
     ('smx_build_library_batch', ctypes.c_int,
      [_I32, ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),

@@ -362,9 +362,14 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
   __shared__ __align__(16) _Float16 Bl[AG_BN][P];
   __shared__ __align__(16) _Float16 Ah[AG_KR][P];
   __shared__ __align__(16) _Float16 Al[AG_KR][P];
-  const int64_t j0 = (int64_t)blockIdx.x * AG_BN;
-  const int split = blockIdx.y;
-  const int64_t k0 = (int64_t)blockIdx.z * AG_KR;
+  // 1-D grid, row block of R fastest (the Kp/128 row blocks that share a tile
+  // of M run back to back and share it through the caches), then column
+  // block, then split; with one row block this is column block, then split.
+  const int64_t nrb = Kp / AG_KR, ncb = npad / AG_BN;
+  const int64_t bid = blockIdx.x;
+  const int64_t k0 = (bid % nrb) * AG_KR;
+  const int64_t j0 = ((bid / nrb) % ncb) * AG_BN;
+  const int split = (int)(bid / (nrb * ncb));
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 5, c = lane & 31;
   const int64_t nch = npad / KC;
@@ -864,7 +869,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   int done = 0;
   const int64_t nch = npad / AG_KP;
   const int cps = static_cast<int>((nch + S - 1) / S);
-  dim3 ggrid((unsigned)(npad / AG_BN), (unsigned)S, (unsigned)(Kp / AG_KR));
+  dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM, st);
   for (int it = 0; it < maxiter; ++it) {
     const bool chk = tol > 0.0 && ((it + 1) % check == 0);
@@ -907,6 +912,116 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w.z, W);
   SMX_LAUNCHED();
   if (iters_done) *iters_done = done;
+  return 0;
+}
+
+}  // extern "C"
+namespace smx {
+// ---------------------------------------------------------------------------
+// The ADMM operator alone: M = (U^T U + (lambda/n) I + rho I)^-1 built once,
+// then applied to any K right-hand sides (K <= the K the workspace was sized
+// for).  Used where many solves share one Gram up to low-rank changes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__ R, int64_t K,
+                                                       int64_t n, int64_t npad,
+                                                       float* __restrict__ Rp,
+                                                       unsigned* __restrict__ rmax) {
+  __shared__ float red[4];
+  const int64_t k = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  float v = 0.f;
+  if (k < K && i < n) v = R[k * n + i];
+  if (i < npad) Rp[k * npad + i] = v;
+  const float m = block_max_256(fabsf(v), red);
+  if (threadIdx.x == 0 && m > 0.f) atomicMax(rmax + k, __float_as_uint(m));
+}
+
+__global__ void apply_out_kernel(const float* __restrict__ part, int S, int64_t Kp, int64_t npad,
+                                 int64_t K, int64_t n, const float* __restrict__ rho_p,
+                                 const unsigned* __restrict__ rmax, float* __restrict__ out) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= K * n) return;
+  const int64_t k = idx / n, i = idx - k * n;
+  float sum = 0.f;
+  for (int s = 0; s < S; ++s) sum += part[((int64_t)s * Kp + k) * npad + i];
+  const float sM = pow2_scale_for(1.f / *rho_p, 13);
+  const float sR = pow2_scale_for(__uint_as_float(rmax[k]), 14);
+  out[idx] = static_cast<float>(static_cast<double>(sum) /
+                                (static_cast<double>(sM) * static_cast<double>(sR)));
+}
+
+static int nnls_build_operator(hipStream_t st, const NnlsWs& w, const double* lib, int64_t ld_lib,
+                               int64_t n, int64_t m, const double* row_scale, double lambda_reg) {
+  const int64_t ldu = align_up(m, 16);
+  stage_begin(ST_GRAM, st);
+  hipLaunchKernelGGL(build_ut_kernel, dim3(blocks_for(n * ldu, 256)), dim3(256), 0, st, lib, ld_lib,
+                     n, m, ldu, row_scale, w.Ut);
+  SMX_LAUNCHED();
+  if (int e = gemm_nt<float>(st, n, n, ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1)) return e;
+  mirror_lower<float>(st, w.Q, n, n);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
+                     lambda_reg / static_cast<double>(n), w.rho64, w.rho);
+  SMX_LAUNCHED();
+  stage_end(ST_GRAM, st, 5);
+  stage_begin(ST_FACTOR, st);
+  if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
+  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(w.npad * w.npad, 256)), dim3(256), 0, st,
+                     w.Q64, n, w.npad, w.rho, w.Mh, w.Ml);
+  SMX_LAUNCHED();
+  stage_end(ST_FACTOR, st, 1);
+  return 0;
+}
+}  // namespace smx
+using namespace smx;
+extern "C" {
+
+int64_t smx_nnls_operator_bytes(int64_t n, int64_t m, int64_t K_max) {
+  if (n <= 0 || m <= 0 || K_max <= 0) return -1;
+  return carve(nullptr, n, m, K_max).bytes;
+}
+
+int smx_nnls_operator(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
+                      const double* row_scale, double lambda_reg, int64_t K_max, void* workspace,
+                      double* rho_out, void* stream) {
+  SMX_REQUIRE(lib && row_scale && workspace && rho_out, "smx_nnls_operator: null pointer");
+  SMX_REQUIRE(n > 0 && m > 0 && K_max > 0 && ld_lib >= m && n < (int64_t)1 << 31 &&
+                  m < (int64_t)1 << 31,
+              "smx_nnls_operator: bad sizes");
+  hipStream_t st = as_stream(stream);
+  NnlsWs w = carve(workspace, n, m, K_max);
+  if (int e = nnls_build_operator(st, w, lib, ld_lib, n, m, row_scale, lambda_reg)) return e;
+  SMX_HIP(hipMemcpyAsync(rho_out, w.rho64, sizeof(double), hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
+                            int64_t K, float* out, void* stream) {
+  SMX_REQUIRE(workspace && R && out, "smx_nnls_operator_apply: null pointer");
+  SMX_REQUIRE(n > 0 && m > 0 && K > 0 && K <= K_max, "smx_nnls_operator_apply: bad sizes");
+  hipStream_t st = as_stream(stream);
+  NnlsWs w = carve(workspace, n, m, K_max);
+  const int64_t npad = w.npad, Kp = align_up(K, AG_KR);
+  const int S = w.S;
+  const int64_t nch = npad / AG_KP;
+  const int cps = static_cast<int>((nch + S - 1) / S);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp, 256)), dim3(256), 0, st, w.rmax, Kp, 0u);
+  hipLaunchKernelGGL(pad_rows_kernel, dim3(blocks_for(npad, 256), (unsigned)Kp), dim3(256), 0, st, R,
+                     K, n, npad, w.Rp, w.rmax);
+  hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
+                     Kp * npad, npad, w.rmax, w.Rh, w.Rl);
+  SMX_LAUNCHED();
+  dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
+  stage_begin(ST_ADMM_PRODUCT, st);
+  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rh, w.Rl,
+                     w.Mh, w.Ml, w.part);
+  SMX_LAUNCHED();
+  stage_end(ST_ADMM_PRODUCT, st, 1);
+  hipLaunchKernelGGL(apply_out_kernel, dim3(blocks_for(K * n, 256)), dim3(256), 0, st, w.part, S,
+                     Kp, npad, K, n, w.rho, w.rmax, out);
+  SMX_LAUNCHED();
   return 0;
 }
 

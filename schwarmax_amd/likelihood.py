@@ -50,6 +50,21 @@ def grid_args(dict_data):
     return Rzphi_lim, np.asarray(dict_data['Rzphi_n_grid']), xy_lim, np.asarray(dict_data['nX_nY'])
 
 
+def chi2_func(y_Rzphi, y_Rzphi_model, sigma_Rzphi,
+              y_xy, y_xy_model, sigma_xy,
+              y_h1, y_h1_model, sigma_h1,
+              y_h2, y_h2_model, sigma_h2,
+              y_h3, y_h3_model, sigma_h3,
+              y_h4, y_h4_model, sigma_h4):
+    """The jackknife's chi^2 (likelihood.py:73-94) on torch tensors; batched
+    over a leading replicate axis like the reference's vmap.  Only the
+    kinematic terms enter the returned value, as in the reference."""
+    def res(y, ym, sg):
+        return torch.nansum(((y - ym) / sg) ** 2, dim=-1)
+    return -0.5 * (res(y_h1, y_h1_model, sigma_h1) + res(y_h2, y_h2_model, sigma_h2) +
+                   res(y_h3, y_h3_model, sigma_h3) + res(y_h4, y_h4_model, sigma_h4))
+
+
 def bootstrap_loglike(model_dict, dict_data, sigma_amplify=1.0):
     """Bootstrap chi^2 rows and their log-mean-exp (likelihood.py:184-244)."""
     from ._lib import dev

@@ -189,13 +189,10 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     return out, aux
 
 
-def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regularization,
-                       channel_w, nnls_tol=0.0):
-    """Everything build_model does after the library (model.py:192-286):
-    density target, data normalisation, bootstrap NNLS and the model maps."""
-    from .nnls import solve_nnls_admm_bootstrap_lib
+def prepare_targets(n, params_disk_rho, dict_data):
+    """Density target and normalised data of model.py:192-258 (mass per
+    orbit, M/L, gamma_kin), as device tensors."""
     lm = 1.0 / float(params_disk_rho['mass_to_light_ratio'])
-    n = lib.shape[0]
     y_R = density_target(params_disk_rho, dict_data)
     sig_R = 0.02 * y_R + 1e-10
     D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
@@ -217,6 +214,21 @@ def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regu
     M_per_bin = y_xy * npb * area
     norm_GH = 1.0 + y_h[3] * np.sqrt(6.0) / 4.0
     gamma_kin = M_per_bin / (norm_GH * 2.0 * np.sqrt(np.pi) * s_ap + EPSILON)
+    return dict(lm=lm, y_R=y_R, sig_R=sig_R, y_xy=y_xy, sig_xy=sig_xy, y_h=y_h, sig_h=sig_h,
+                mmpo=mmpo, gamma_kin=gamma_kin, s_ap=s_ap, v0_ap=v0_ap)
+
+
+def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regularization,
+                       channel_w, nnls_tol=0.0):
+    """Everything build_model does after the library (model.py:192-286):
+    density target, data normalisation, bootstrap NNLS and the model maps."""
+    from .nnls import solve_nnls_admm_bootstrap_lib
+    n = lib.shape[0]
+    t = prepare_targets(n, params_disk_rho, dict_data)
+    lm, y_R, sig_R, y_xy, sig_xy = t['lm'], t['y_R'], t['sig_R'], t['y_xy'], t['sig_xy']
+    y_h, sig_h, mmpo, gamma_kin = t['y_h'], t['sig_h'], t['mmpo'], t['gamma_kin']
+    s_ap, v0_ap = t['s_ap'], t['v0_ap']
+    D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
     y_xy_boot = y_xy[None, :] + D('XY_standard_normal') * sig_xy[None, :]
     y_h_boot = [y_h[k][None, :] + D('h%d_standard_normal' % (k + 1)) * sig_h[k][None, :]
                 for k in range(4)]

@@ -35,6 +35,8 @@ CONFIGS = {
                        n_models=64),
     'grad-20k': dict(n=20000, N_max=2000, N_dyn=100, rtol=1e-4, n_ap=1000, field_arcsec=60,
                      grid=(32, 16, 24), K=100, T=1000, seeded_params=True),
+    'jackknife-20k': dict(n=20000, N_max=2000, N_dyn=100, rtol=1e-4, n_ap=1000, field_arcsec=60,
+                          grid=(32, 16, 24), K=100, T=200, seeded_params=True),
     'batched-grad-16': dict(n=40000, N_max=2000, N_dyn=100, rtol=1e-4, n_ap=2000,
                             field_arcsec=80, grid=(40, 20, 32), K=200, T=1000,
                             seeded_params=True, n_models=16),

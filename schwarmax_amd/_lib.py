@@ -163,7 +163,10 @@ def dev(a, dtype=torch.float64, device='cuda'):
     """Host or device array -> contiguous device tensor of dtype."""
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+    a = np.ascontiguousarray(a)
+    if not a.flags.writeable:
+        a = a.copy()
+    return torch.as_tensor(a, dtype=dtype, device=device)
 
 
 STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps', 'density',

@@ -76,8 +76,7 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
     rows = P + 6 * B
     dev = _lib.dev
     w0 = dev(w0_all)
-    normals = dev(dict_data['jeans_normals'])
-    uni = dev(dict_data['jitter_uniforms'])
+    normals, uni = (dev(a) for a in MD.ic_draws(dict_data, r))
     r_total = uni.shape[1]
     bm = dev(dict_data['bin_mapping'], dtype=torch.int32)
     npb, v0, sd = dev(dict_data['num_per_bin']), dev(dict_data['v0']), dev(dict_data['s'])

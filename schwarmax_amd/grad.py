@@ -121,8 +121,7 @@ class _InitialConditions(torch.autograd.Function):
         dd, r = ctx.dd, ctx.r
         w0 = _lib.dev(np.asarray(dd['w0'])[:, :3])
         n = w0.shape[0]
-        normals = _lib.dev(dd['jeans_normals'])
-        uni = _lib.dev(dd['jitter_uniforms'])
+        normals, uni = (_lib.dev(a) for a in MD.ic_draws(dd, r))
         jw = torch.empty((n, r, 6, N_FIELDS), dtype=torch.float64, device='cuda')
         jd = torch.empty((n, r, N_FIELDS), dtype=torch.float64, device='cuda')
         _lib.check(L.smx_initial_conditions_jac(ctx.pot, _lib.ptr(w0), _lib.ptr(normals),

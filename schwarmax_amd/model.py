@@ -6,9 +6,10 @@ per-cell and per-iteration computation runs in the HIP kernels of libsmx.so;
 torch holds device memory and does the few O(K*B) elementwise steps between
 kernels (data normalisation, clipping, V/sigma).
 
-Two inputs the reference draws internally from fixed JAX PRNG keys
-(model.py:112, 128-138) are explicit here, because JAX's threefry stream
-cannot be reproduced without JAX: ``dict_data['jeans_normals']`` (n, 3) and
+The reference draws the Jeans normals and jitter uniforms inside build_model
+from fixed JAX PRNG keys (model.py:112, 128-138).  Here they are the same
+draws, from the NumPy restatement of JAX's generator in ``prng``, unless
+dict_data carries its own as ``dict_data['jeans_normals']`` (n, 3) and
 ``dict_data['jitter_uniforms']`` (n, r, 6).  The integration schedule that
 build_model hard-codes (model.py:163-169) is the default of ``schedule``.
 """
@@ -17,6 +18,7 @@ import torch
 
 from . import _lib
 from . import potentials as POT
+from . import prng
 from .utils import makeRotationMatrix
 from .ghmoments import h_to_V_sigma
 
@@ -78,13 +80,35 @@ def make_schedule(s, Omega):
     return sc
 
 
+def ic_draws(dict_data, r):
+    """Jeans normals (n, 3) and jitter uniforms (n, r_total >= r, 6).
+
+    dict_data's 'jeans_normals' and 'jitter_uniforms' when it has them;
+    otherwise the reference's own draws (model.py:112-138, jeans.py:89: keys
+    42, 109, 2026 and split(911, 6), in JAX's partitionable threefry layout)
+    for max(r, 4) realizations, the reference's 4 first.
+    """
+    has = ('jeans_normals' in dict_data, 'jitter_uniforms' in dict_data)
+    if all(has):
+        normals, uni = dict_data['jeans_normals'], dict_data['jitter_uniforms']
+    elif any(has):
+        raise ValueError("dict_data must hold both 'jeans_normals' and 'jitter_uniforms' or neither")
+    else:
+        n = np.asarray(dict_data['w0']).shape[0]
+        normals, uni = prng.reference_draws(n, max(int(r), 4))
+    n = np.asarray(dict_data['w0']).shape[0]
+    if tuple(normals.shape) != (n, 3) or len(uni.shape) != 3 or uni.shape[0] != n \
+            or uni.shape[2] != 6 or uni.shape[1] < r:
+        raise ValueError("draws must be (n, 3) normals and (n, >= r, 6) uniforms for n = %d" % n)
+    return normals, uni
+
+
 def initial_conditions(pot, dict_data, r, sched):
     """Jeans ICs, jitter and time scales on the GPU (model.py:112-169)."""
     L = _lib.lib()
     w0 = _lib.dev(np.asarray(dict_data['w0'])[:, :3])
     n = w0.shape[0]
-    normals = _lib.dev(dict_data['jeans_normals'])
-    uni = _lib.dev(dict_data['jitter_uniforms'])
+    normals, uni = (_lib.dev(a) for a in ic_draws(dict_data, r))
     r_total = uni.shape[1]
     wb = torch.empty((n, r, 6), dtype=torch.float64, device='cuda')
     Tt = torch.empty((n, r), dtype=torch.float64, device='cuda')

@@ -6,9 +6,10 @@ sentinel after the last pixel, the (R, z, phi) cell-centre arrays sorted by
 cell index, the unscrambled 1024-point Sobol set, and bootstrap normals drawn
 exactly as utils.py:604-618 draws them (numpy default_rng(42), row 0 zeroed).
 The JAX-keyed draws the reference makes inside build_model (Jeans normals,
-jitter uniforms; model.py:112, 128-138) are drawn here from a seeded NumPy
-generator and carried in ``dict_data['jeans_normals']`` and
-``dict_data['jitter_uniforms']``.
+jitter uniforms; model.py:112, 128-138) are drawn here from the galaxy's
+seeded NumPy generator and carried in ``dict_data['jeans_normals']`` and
+``dict_data['jitter_uniforms']``, so that every seed gives other draws;
+without those keys the model takes the reference's own streams.
 
 Observations (surface density and h1..h4) are made by ``observe`` from an orbit
 library and a seeded Dirichlet weight vector, plus Gaussian noise.
@@ -17,6 +18,8 @@ import math
 
 import numpy as np
 from scipy.stats import qmc
+
+from .data import rzphi_cells, bootstrap_normals  # noqa: F401  (shared with the loader)
 
 KMS_TO_KPCGYR = 1.022712165045695
 ARCSEC_RAD = math.pi / (180.0 * 3600.0)
@@ -100,26 +103,6 @@ def voronoi_apertures(nX, nY, n_ap, rng):
     return np.concatenate([owner, np.array([-1], dtype=np.int32)]), num
 
 
-def rzphi_cells(nR, nz, nphi, Rmax=10.0, zmax=3.0):
-    """Cell centres sorted by cell index, as utils.py:627-657 builds them."""
-    R_edge = np.linspace(0.0, Rmax, nR + 1)
-    z_edge = np.linspace(-zmax, zmax, nz + 1)
-    p_edge = np.linspace(-np.pi, np.pi, nphi + 1)
-    mids = [0.5 * (e[:-1] + e[1:]) for e in (R_edge, z_edge, p_edge)]
-    Rm, zm, pm = np.meshgrid(*mids, indexing='ij')
-    g = np.stack([Rm.ravel(), zm.ravel(), pm.ravel()], axis=-1)
-    lo = np.array([0.0, -zmax, -np.pi])
-    hi = np.array([Rmax, zmax, np.pi])
-    n = np.array([nR, nz, nphi])
-    q = (g - lo) / (hi - lo)
-    c = np.clip(np.floor(q * n).astype(np.int64), 0, n - 1)
-    idx = c[:, 0] + c[:, 1] * nR + c[:, 2] * nR * nz
-    order = np.argsort(idx)
-    Rg, zg, pg = g[order, 0], g[order, 1], g[order, 2]
-    d = [np.unique(a)[1] - np.unique(a)[0] for a in (Rg, zg, pg)]
-    return Rg, zg, pg, d[0], d[1], d[2]
-
-
 def disc_positions(n, rng, R_d=3.0, h_z=0.3, R_max=12.0):
     """Double-exponential disc: R ~ R exp(-R/R_d) (truncated), |z| ~ exp(-|z|/h_z)."""
     R = np.empty(0)
@@ -171,16 +154,6 @@ def make_galaxy(config='plain-2k', seed=0, n=None, n_ap=None, grid=None, K=None,
                  rtol=cfg['rtol'], atol=1e-7, dt_min=1e-5, dt_max=0.3, chunk=100,
                  n_realizations=r)
     return theta, dd, sched
-
-
-def bootstrap_normals(dd, K, B):
-    """Bootstrap standard normals exactly as utils.py:604-618 draws them."""
-    rng = np.random.default_rng(42)
-    names = ['XY', 'h1', 'h2', 'h3', 'h4', 'V', 'sigma']
-    for nm in names:
-        a = rng.normal(size=(K, B))
-        a[0, :] = 0.0
-        dd[nm + '_standard_normal'] = a
 
 
 def sky_projection(w, R):

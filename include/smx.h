@@ -258,6 +258,49 @@ int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream);
 int smx_model_maps(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
                    const double* W, int64_t K, double* maps, void* stream);
 
+/* ------------------------------------------------------------------------
+ * CylSpline potential (CylindricalSpline.py:269-536, CubicSpline.py).
+ * ---------------------------------------------------------------------- */
+#define SMX_CYLSPLINE_MAX_MODES 21
+
+/* Mode tables of a CylSpline potential on the scaled grid u_i = asinh(R_i/R0),
+ * v_j = asinh(z_j/R0).  coef[((i*nZ + j)*n_modes + m)*6 + k] holds, for mode m
+ * at node (i, j), k = 0..5: Phi_m real, imaginary, their second derivatives
+ * along u (the spline's M_x), then along v (M_y).  The reference keeps the
+ * same numbers as separate (n_modes, nR, nZ) arrays (Phi_m_real, Phi_m_img,
+ * Mx_real, Mx_img, My_real, My_img; CylindricalSpline.py:339-350).
+ * n_modes_eval <= n_modes modes are summed (1: the axisymmetric part). */
+typedef struct smx_cylspline {
+  const double* u;
+  const double* v;
+  const double* coef;
+  int32_t nR, nZ, n_modes, n_modes_eval;
+  double R0;
+  double G;
+} smx_cylspline;
+
+/* The builder's quadrature (CylindricalSpline.py:248-268, 318-330):
+ * phi_m[(2m + part)*nR*nZ + i*nZ + j] = -G sum_k node_w[k]
+ *   rho_nodes[(2m + part)*n_nodes + k] (Xi_m(R_i, Z_j; node_R[k], node_z[k])
+ *   + Xi_m(R_i, Z_j; node_R[k], -node_z[k])), part 0 real, 1 imaginary;
+ * node_w holds the Simpson weight times 2 pi R' times the map's Jacobian.
+ * Xi_m = Q_{m-1/2}(chi)/(pi sqrt(R R')) as in jax_kernel_Xi_m
+ * (CylindricalSpline.py:162-190). */
+int smx_cylspline_phi_m(int32_t n_modes, const double* R, int32_t nR, const double* Z, int32_t nZ,
+                        const double* node_R, const double* node_z, const double* node_w,
+                        int64_t n_nodes, const double* rho_nodes, double G, double* phi_m,
+                        void* stream);
+
+/* Phi (n), acceleration -grad Phi (n, 3) and density nabla^2 Phi/(4 pi G) (n)
+ * at points xyz (n, 3); any output may be NULL.  Replaces evaluate_phi,
+ * evaluate_phi_axisymmetric, get_acc and get_density
+ * (CylindricalSpline.py:355-416, 489-536). */
+int smx_cylspline_eval(const smx_cylspline* t, const double* xyz, int64_t n, double* phi,
+                       double* acc, double* rho, void* stream);
+
+/* Test hook: Q_{m-1/2}(1 + delta) (jax_legendreQ, CylindricalSpline.py:145-160). */
+int smx_legendre_q_half(int32_t m, const double* delta, int64_t n, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,16 @@ class SmxSchedule(ctypes.Structure):
     ]
 
 
+class SmxCylspline(ctypes.Structure):
+    _fields_ = [
+        ('u', ctypes.c_void_p), ('v', ctypes.c_void_p), ('coef', ctypes.c_void_p),
+        ('nR', ctypes.c_int32), ('nZ', ctypes.c_int32), ('n_modes', ctypes.c_int32),
+        ('n_modes_eval', ctypes.c_int32), ('R0', ctypes.c_double), ('G', ctypes.c_double),
+    ]
+
+
+CYLSPLINE_MAX_MODES = 21
+
 # (name, restype, argtypes) of every entry point in include/smx.h
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -83,6 +93,9 @@ _SIGS = [
     ('smx_nnls_operator_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_operator', ctypes.c_int, [_P, _I64, _I64, _I64, _P, ctypes.c_double, _I64, _P, _P, _P]),
     ('smx_nnls_operator_apply', ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, _P, _P]),
+    ('smx_cylspline_phi_m', ctypes.c_int, [_I32, _P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _D, _P, _P]),
+    ('smx_cylspline_eval', ctypes.c_int, [ctypes.POINTER(SmxCylspline), _P, _I64, _P, _P, _P, _P]),
+    ('smx_legendre_q_half', ctypes.c_int, [_I32, _P, _I64, _P, _P]),
     ('smx_batch_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I32, _I64]),
     ('smx_build_library_batch', ctypes.c_int,
      [_I32, ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),

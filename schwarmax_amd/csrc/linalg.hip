@@ -23,6 +23,9 @@
 #include <cstdlib>
 #include <algorithm>
 
+#include <mutex>
+#include <rocblas/rocblas.h>
+
 #include "common.hpp"
 #include "timing.hpp"
 #include "../../include/smx.h"
@@ -279,32 +282,46 @@ __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
   return ldexpf(1.f, target_exp - e);
 }
 
-// M (n, n) FP64 -> scaled fp16 hi / lo (npad, npad), zero padded.
+// Chunk-blocked layouts of the split planes, so that every (column block,
+// chunk) tile a workgroup loads is one contiguous run:
+//   M: tile (J, c) = M[256 J + r][32 c + q], r < 256, q < 32, at
+//      ((J * nch + c) * 256 + r) * 32 + q   (nch = npad / 32);
+//   R: tile c = R[k][32 c + q], k < Kp, at (c * Kp + k) * 32 + q.
+__device__ __forceinline__ int64_t m_tile_index(int64_t j, int64_t i, int64_t nch) {
+  return ((j / AG_BN * nch + i / AG_KP) * AG_BN + j % AG_BN) * AG_KP + i % AG_KP;
+}
+__device__ __forceinline__ int64_t r_tile_index(int64_t k, int64_t i, int64_t Kp) {
+  return ((i / AG_KP) * Kp + k) * AG_KP + i % AG_KP;
+}
+
+// M (n, n) FP64 -> scaled fp16 hi / lo (npad x npad, blocked), zero padded.
 __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t npad,
                                const float* __restrict__ rho_p, _Float16* __restrict__ Mh,
                                _Float16* __restrict__ Ml) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= npad * npad) return;
-  const int64_t i = idx / npad, j = idx - i * npad;
+  const int64_t j = idx / npad, i = idx - j * npad;
   const double sc = pow2_scale_for(1.f / *rho_p, 13);   // power of two; |M_ij| <= 1/rho
-  const double v = (i < n && j < n) ? M[i * n + j] * sc : 0.0;
+  const double v = (i < n && j < n) ? M[j * n + i] * sc : 0.0;
   const _Float16 h = static_cast<_Float16>(v);
-  Mh[idx] = h;
-  Ml[idx] = static_cast<_Float16>(v - static_cast<double>(h));
+  const int64_t d = m_tile_index(j, i, npad / AG_KP);
+  Mh[d] = h;
+  Ml[d] = static_cast<_Float16>(v - static_cast<double>(h));
 }
 
-// R (FP32, Kp x npad) -> scaled fp16 hi / lo with each row's power-of-two
-// scale (max |R_k| -> 2^14).
-__global__ void split_r_kernel(const float* __restrict__ R, int64_t total, int64_t npad,
+// R (FP32, Kp x npad) -> scaled fp16 hi / lo (blocked) with each row's
+// power-of-two scale (max |R_k| -> 2^14).
+__global__ void split_r_kernel(const float* __restrict__ R, int64_t Kp, int64_t npad,
                                const unsigned* __restrict__ rowmax, _Float16* __restrict__ Rh,
                                _Float16* __restrict__ Rl) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const int64_t k = idx / npad;
+  if (idx >= Kp * npad) return;
+  const int64_t k = idx / npad, i = idx - k * npad;
   const float v = R[idx] * pow2_scale_for(__uint_as_float(rowmax[k]), 14);
   const _Float16 h = static_cast<_Float16>(v);
-  Rh[idx] = h;
-  Rl[idx] = static_cast<_Float16>(v - static_cast<float>(h));
+  const int64_t d = r_tile_index(k, i, Kp);
+  Rh[d] = h;
+  Rl[d] = static_cast<_Float16>(v - static_cast<float>(h));
 }
 
 // One chunk (KC reduction indices) of the ADMM product's operands.  B: 256
@@ -316,34 +333,44 @@ __global__ void split_r_kernel(const float* __restrict__ R, int64_t total, int64
 // workgroups per CU.
 template <int KC>
 struct AdmmStage {
+  static_assert(KC == AG_KP, "the blocked split layouts use 32-wide chunks");
   static constexpr int TPR = KC / 8;
   static constexpr int RPP = 256 / TPR;
   static constexpr int PB_ = AG_BN / RPP;   // passes for B
   static constexpr int PA_ = AG_KR / RPP;   // passes for A
   uint4 bh[PB_], bl[PB_];
+  // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
+  // row t/4 + 64p, segment t%4, i.e. the tile's uint4 number t + 256p
   __device__ void load_b(const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml,
                          int64_t npad, int64_t j0, int64_t i0) {
     const int t = threadIdx.x;
+    const int64_t base = ((j0 / AG_BN) * (npad / AG_KP) + i0 / AG_KP) * (int64_t)(AG_BN * AG_KP);
+    // M streams once per launch: non-temporal loads keep it from evicting the
+    // R tiles that the workgroups of one split share in L2.
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int p = 0; p < PB_; ++p) {
-      const int64_t off = (j0 + t / TPR + RPP * p) * npad + i0 + 8 * (t % TPR);
-      bh[p] = *reinterpret_cast<const uint4*>(Mh + off);
-      bl[p] = *reinterpret_cast<const uint4*>(Ml + off);
+      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
+      const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Mh + off));
+      const u32x4 l = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Ml + off));
+      bh[p] = make_uint4(h.x, h.y, h.z, h.w);
+      bl[p] = make_uint4(l.x, l.y, l.z, l.w);
     }
   }
   template <int P>
   __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P], _Float16 (*Al)[P],
                         const _Float16* __restrict__ Rh, const _Float16* __restrict__ Rl,
-                        int64_t npad, int64_t k0, int64_t i0) const {
+                        int64_t Kp, int64_t k0, int64_t i0) const {
     const int t = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PB_; ++p) {
       *reinterpret_cast<uint4*>(&Bh[t / TPR + RPP * p][8 * (t % TPR)]) = bh[p];
       *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
     }
+    const int64_t base = ((i0 / AG_KP) * Kp + k0) * AG_KP;
 #pragma unroll
     for (int p = 0; p < PA_; ++p) {
-      const int64_t off = (k0 + t / TPR + RPP * p) * npad + i0 + 8 * (t % TPR);
+      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
       *reinterpret_cast<uint4*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) =
           *reinterpret_cast<const uint4*>(Rh + off);
       *reinterpret_cast<uint4*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) =
@@ -385,7 +412,7 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
   AdmmStage<KC> stg;
   if (ch0 < ch1) {
     stg.load_b(Mh, Ml, npad, j0, ch0 * KC);
-    stg.store(Bh, Bl, Ah, Al, Rh, Rl, npad, k0, ch0 * KC);
+    stg.store(Bh, Bl, Ah, Al, Rh, Rl, Kp, k0, ch0 * KC);
   }
   __syncthreads();
   for (int64_t ch = ch0; ch < ch1; ++ch) {
@@ -413,7 +440,7 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
     __syncthreads();
     if (more) {
       stg.load_b(Mh, Ml, npad, j0, (ch + 1) * KC);
-      stg.store(Bh, Bl, Ah, Al, Rh, Rl, npad, k0, (ch + 1) * KC);
+      stg.store(Bh, Bl, Ah, Al, Rh, Rl, Kp, k0, (ch + 1) * KC);
     }
     __syncthreads();
   }
@@ -732,6 +759,49 @@ static void mirror_lower(hipStream_t st, T* A, int64_t n, int64_t lda) {
 //   Tt = X A21^T,  T = Tt^T,  A22 -= T A21^T (lower),  A21 = -S^-1 Tt^T,
 //   A12 = A21^T,  A11 -= Tt A12^T (lower).  work holds 2 n1 n2 per level, n^2
 // in total suffices.
+// ---------------------------------------------------------------------------
+// The inverse's products are plain FP64 GEMMs, run by rocBLAS (its library
+// kernels reach 0.83-0.92 of the FP64 MFMA peak at n = 5k-10k here, against
+// about 0.5 for gemm_nt_kernel<double>).  Row-major X is the column-major X^T,
+// so C = alpha A B^T + beta C (row-major, A: M x K, B: N x K) is the
+// column-major C^T = alpha (B^T)^T A^T + beta C^T: dgemm(T, N, N, M, K) with
+// the operands swapped.  A product known to be symmetric updates only the
+// row-major lower triangle, the column-major upper one: dsyrkx(upper, T).
+// One handle per device, created on first use.
+// ---------------------------------------------------------------------------
+static rocblas_handle blas_handle(hipStream_t st) {
+  static std::mutex mu;
+  static rocblas_handle handles[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) {
+    handles[dev] = nullptr;
+    return nullptr;
+  }
+  if (rocblas_set_stream(handles[dev], st) != rocblas_status_success) return nullptr;
+  return handles[dev];
+}
+
+static int dgemm_nt_blas(hipStream_t st, int64_t M, int64_t N, int64_t K, double alpha,
+                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                         double* C, int64_t ldc, int symmetric_lower = 0) {
+  if (M <= 0 || N <= 0) return 0;
+  rocblas_handle h = blas_handle(st);
+  SMX_REQUIRE(h, "rocBLAS handle could not be created");
+  rocblas_status r;
+  if (symmetric_lower)
+    r = rocblas_dsyrkx(h, rocblas_fill_upper, rocblas_operation_transpose, (rocblas_int)N,
+                       (rocblas_int)K, &alpha, B, (rocblas_int)ldb, A, (rocblas_int)lda, &beta, C,
+                       (rocblas_int)ldc);
+  else
+    r = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)N,
+                      (rocblas_int)M, (rocblas_int)K, &alpha, B, (rocblas_int)ldb, A,
+                      (rocblas_int)lda, &beta, C, (rocblas_int)ldc);
+  SMX_REQUIRE(r == rocblas_status_success, "rocBLAS FP64 product failed (status %d)", (int)r);
+  return 0;
+}
+
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
     hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(64), 0, st, A, lda, (int)n);
@@ -749,18 +819,18 @@ static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double
   double* deeper = work + 2 * n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
   mirror_lower(st, A11, n1, lda);
-  if (int e = gemm_nt<double>(st, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, Tt, n2)) return e;
+  if (int e = dgemm_nt_blas(st, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, Tt, n2)) return e;
   hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
                      Tt, n2, n1, n2, Tm, n1);
   SMX_LAUNCHED();
-  if (int e = gemm_nt<double>(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda, 1)) return e;
+  if (int e = dgemm_nt_blas(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda, 1)) return e;
   if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
   mirror_lower(st, A22, n2, lda);
-  if (int e = gemm_nt<double>(st, n2, n1, n2, -1.0, A22, lda, Tt, n2, 0.0, A21, lda)) return e;
+  if (int e = dgemm_nt_blas(st, n2, n1, n2, -1.0, A22, lda, Tt, n2, 0.0, A21, lda)) return e;
   hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
                      A21, lda, n2, n1, A12, lda);
   SMX_LAUNCHED();
-  return gemm_nt<double>(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda, 1);
+  return dgemm_nt_blas(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda, 1);
 }
 
 // Full inverse of the SPD matrix whose lower triangle is given.
@@ -879,7 +949,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     unsigned* rnext = w.rmax + ((it + 1) % 3) * Kp;
     unsigned* rclear = w.rmax + ((it + 2) % 3) * Kp;
     hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
-                       Kp * npad, npad, rcur, w.Rh, w.Rl);
+                       Kp, npad, rcur, w.Rh, w.Rl);
     stage_begin(ST_ADMM_PRODUCT, st);
     hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rh,
                        w.Rl, w.Mh, w.Ml, w.part);
@@ -1011,7 +1081,7 @@ int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace
   hipLaunchKernelGGL(pad_rows_kernel, dim3(blocks_for(npad, 256), (unsigned)Kp), dim3(256), 0, st, R,
                      K, n, npad, w.Rp, w.rmax);
   hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
-                     Kp * npad, npad, w.rmax, w.Rh, w.Rl);
+                     Kp, npad, w.rmax, w.Rh, w.Rl);
   SMX_LAUNCHED();
   dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM_PRODUCT, st);

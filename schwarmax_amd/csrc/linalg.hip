@@ -457,20 +457,31 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
       }
 }
 
-// init: z = w_init, u = 0, R = rho (z - u) - c, row maxima of |R|
-__global__ void admm_init2_kernel(int64_t K, int64_t n, int64_t npad, double w_init,
-                                  const double* __restrict__ rho_p, const float* __restrict__ c,
-                                  double* __restrict__ z, double* __restrict__ u,
-                                  float* __restrict__ R, unsigned* __restrict__ rowmax) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= K * n) return;
-  const int64_t k = idx / n, i = idx - k * n;
-  const double rho = *rho_p;
-  z[idx] = w_init;
-  u[idx] = 0.0;
-  const float r = static_cast<float>(rho * (w_init - 0.0) - static_cast<double>(c[idx]));
-  R[k * npad + i] = r;
-  atomicMax(rowmax + k, __float_as_uint(fabsf(r)));   // once per element, once per solve
+__device__ __forceinline__ float block_max_256(float v, float* red);
+
+// init: z = w_init, u = 0, R = rho (z - u) - c, row maxima of |R|.  Grid
+// (ceil(n / 256), K): one row per blockIdx.y, one atomic per block.
+__global__ __launch_bounds__(256) void admm_init2_kernel(int64_t K, int64_t n, int64_t npad,
+                                                         double w_init,
+                                                         const double* __restrict__ rho_p,
+                                                         const float* __restrict__ c,
+                                                         double* __restrict__ z, double* __restrict__ u,
+                                                         float* __restrict__ R,
+                                                         unsigned* __restrict__ rowmax) {
+  __shared__ float red[4];
+  const int64_t k = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  float m = 0.f;
+  if (i < n) {
+    const int64_t idx = k * n + i;
+    z[idx] = w_init;
+    u[idx] = 0.0;
+    const float r = static_cast<float>(*rho_p * (w_init - 0.0) - static_cast<double>(c[idx]));
+    R[k * npad + i] = r;
+    m = fabsf(r);
+  }
+  m = block_max_256(m, red);
+  if (threadIdx.x == 0 && m > 0.f) atomicMax(rowmax + k, __float_as_uint(m));
 }
 
 __device__ __forceinline__ float block_max_256(float v, float* red) {
@@ -484,9 +495,24 @@ __device__ __forceinline__ float block_max_256(float v, float* red) {
 }
 
 // ADMM step on the summed partial products (nnls.py:152-157), next R and its
-// row maxima.  Grid (ceil(n / 256), K): one row per blockIdx.y, one atomic per
-// block.  rowmax_cur scales this iteration's product, rowmax_next collects the
-// next one, rowmax_clear is reset for the iteration after.
+// row maxima.  Grid (ceil(n / (256 V)), K): one row per blockIdx.y, V
+// consecutive elements per thread (V = 4 needs n % 4 == 0, so that every
+// vector access is aligned), one atomic per block.  rowmax_cur scales this
+// iteration's product, rowmax_next collects the next one, rowmax_clear is
+// reset for the iteration after.
+struct AdmmStep {
+  double rho, alpha, inv_scale;
+  __device__ __forceinline__ void apply(float sum, double zo, double uo, float cv, double& zn,
+                                        double& un, float& r) const {
+    const double w = static_cast<double>(sum) * inv_scale;
+    const double wh = alpha * w + (1.0 - alpha) * zo;
+    zn = fmax(0.0, wh + uo);
+    un = (uo + wh) - zn;
+    r = static_cast<float>(rho * (zn - un) - static_cast<double>(cv));
+  }
+};
+
+template <int V>
 __global__ __launch_bounds__(256) void admm_update_kernel(
     int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
     const double* __restrict__ rho_p, const float* __restrict__ c, double* __restrict__ z,
@@ -495,34 +521,61 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
     const int* __restrict__ active, unsigned* __restrict__ delta) {
   __shared__ float red[4];
   const int64_t k = blockIdx.y;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * V;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     rowmax_clear[k] = 0u;
     if (k == K - 1)
       for (int64_t q = K; q < Kp; ++q) rowmax_clear[q] = 0u;
   }
-  const int64_t o = k * npad + i;
+  const int64_t o = k * npad + i0;
   float rmax = 0.f, dmax = 0.f;
   if (!active[k]) {
-    if (i < n) rmax = fabsf(R[o]);
-  } else if (i < n) {
-    const int64_t idx = k * n + i;
-    float sum = 0.f;
-    for (int s = 0; s < S; ++s) sum += part[((int64_t)s * Kp + k) * npad + i];
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+      if (i0 + e < n) rmax = fmaxf(rmax, fabsf(R[o + e]));
+  } else if (i0 < n) {
+    const int64_t idx = k * n + i0;
     const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
     const float sR = pow2_scale_for(__uint_as_float(rowmax_cur[k]), 14);
-    const double w = static_cast<double>(sum) / (static_cast<double>(sM) * static_cast<double>(sR));
-    const double rho = *rho_p;
-    const double zo = z[idx], uo = u[idx];
-    const double wh = alpha * w + (1.0 - alpha) * zo;
-    const double zn = fmax(0.0, wh + uo);
-    const double un = (uo + wh) - zn;
-    z[idx] = zn;
-    u[idx] = un;
-    const float r = static_cast<float>(rho * (zn - un) - static_cast<double>(c[idx]));
-    R[o] = r;
-    rmax = fabsf(r);
-    dmax = static_cast<float>(fabs(zn - zo));
+    const AdmmStep st{*rho_p, alpha, 1.0 / (static_cast<double>(sM) * static_cast<double>(sR))};
+    if (V == 4) {
+      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int s = 0; s < S; ++s) {
+        const float4 p4 = *reinterpret_cast<const float4*>(part + ((int64_t)s * Kp + k) * npad + i0);
+        sum.x += p4.x; sum.y += p4.y; sum.z += p4.z; sum.w += p4.w;
+      }
+      const double2 z01 = *reinterpret_cast<const double2*>(z + idx);
+      const double2 z23 = *reinterpret_cast<const double2*>(z + idx + 2);
+      const double2 u01 = *reinterpret_cast<const double2*>(u + idx);
+      const double2 u23 = *reinterpret_cast<const double2*>(u + idx + 2);
+      const float4 c4 = *reinterpret_cast<const float4*>(c + idx);
+      double zn[4], un[4];
+      float r[4];
+      st.apply(sum.x, z01.x, u01.x, c4.x, zn[0], un[0], r[0]);
+      st.apply(sum.y, z01.y, u01.y, c4.y, zn[1], un[1], r[1]);
+      st.apply(sum.z, z23.x, u23.x, c4.z, zn[2], un[2], r[2]);
+      st.apply(sum.w, z23.y, u23.y, c4.w, zn[3], un[3], r[3]);
+      *reinterpret_cast<double2*>(z + idx) = make_double2(zn[0], zn[1]);
+      *reinterpret_cast<double2*>(z + idx + 2) = make_double2(zn[2], zn[3]);
+      *reinterpret_cast<double2*>(u + idx) = make_double2(un[0], un[1]);
+      *reinterpret_cast<double2*>(u + idx + 2) = make_double2(un[2], un[3]);
+      *reinterpret_cast<float4*>(R + o) = make_float4(r[0], r[1], r[2], r[3]);
+      rmax = fmaxf(fmaxf(fabsf(r[0]), fabsf(r[1])), fmaxf(fabsf(r[2]), fabsf(r[3])));
+      dmax = static_cast<float>(fmax(fmax(fabs(zn[0] - z01.x), fabs(zn[1] - z01.y)),
+                                     fmax(fabs(zn[2] - z23.x), fabs(zn[3] - z23.y))));
+    } else {
+      float sum = 0.f;
+      for (int s = 0; s < S; ++s) sum += part[((int64_t)s * Kp + k) * npad + i0];
+      const double zo = z[idx], uo = u[idx];
+      double zn, un;
+      float r;
+      st.apply(sum, zo, uo, c[idx], zn, un, r);
+      z[idx] = zn;
+      u[idx] = un;
+      R[o] = r;
+      rmax = fabsf(r);
+      dmax = static_cast<float>(fabs(zn - zo));
+    }
   }
   rmax = block_max_256(rmax, red);
   __syncthreads();
@@ -928,8 +981,8 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(3 * Kp, 256)), dim3(256), 0, st, w.rmax,
                      3 * Kp, 0u);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(admm_init2_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, K, n, npad,
-                     w_init, w.rho64, w.C, w.z, w.u, w.Rp, w.rmax);
+  hipLaunchKernelGGL(admm_init2_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st, K,
+                     n, npad, w_init, w.rho64, w.C, w.z, w.u, w.Rp, w.rmax);
   SMX_LAUNCHED();
   stage_end(ST_RHS, st, 5);
   std::vector<int> act(K, 1);
@@ -955,9 +1008,14 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
                        w.Rl, w.Mh, w.Ml, w.part);
     SMX_LAUNCHED();
     stage_end(ST_ADMM_PRODUCT, st, 1);
-    hipLaunchKernelGGL(admm_update_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
-                       K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext, rclear,
-                       1.6, w.active, w.delta);
+    if (n % 4 == 0)
+      hipLaunchKernelGGL(admm_update_kernel<4>, dim3(blocks_for(n / 4, 256), (unsigned)K), dim3(256),
+                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext,
+                         rclear, 1.6, w.active, w.delta);
+    else
+      hipLaunchKernelGGL(admm_update_kernel<1>, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0,
+                         st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext,
+                         rclear, 1.6, w.active, w.delta);
     SMX_LAUNCHED();
     done = it + 1;
     if (chk) {

@@ -711,15 +711,6 @@ __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
   if (i < n) p[i] = v;
 }
 
-// dst (cols x rows) = src (rows x cols)^T
-template <class T>
-__global__ void transpose_into_kernel(const T* __restrict__ src, int64_t lds, int64_t rows,
-                                      int64_t cols, T* __restrict__ dst, int64_t ldd) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= rows * cols) return;
-  const int64_t r = idx / cols, c = idx - r * cols;
-  dst[c * ldd + r] = src[r * lds + c];
-}
 
 // ---------------------------------------------------------------------------
 // SPD inverse in FP64 by recursive Schur complements, in place:
@@ -728,52 +719,78 @@ __global__ void transpose_into_kernel(const T* __restrict__ src, int64_t lds, in
 // The recursion reads and writes the lower triangle: the two symmetric
 // updates (S and the new A11) are formed on their lower triangles only, and
 // X and S^-1 are mirrored before they serve as full GEMM operands; the caller
-// mirrors the final lower triangle.  n^3 flops in FP64 MFMA GEMMs (4n^3/3
-// with full updates); 64 x 64 base blocks by Gauss-Jordan in one wavefront
-// (SPD: no pivoting).  The Schur complements must stay positive definite: in
-// FP32 their error, ~ eps cond(A11) |A21|^2, destroys that at the library's
-// conditioning, in FP64 it is ~1e-9 of the smallest eigenvalue.
+// mirrors the final lower triangle.  n^3 flops in FP64 GEMMs (4n^3/3 with
+// full updates); 128 x 128 base blocks by the symmetric sweep in one
+// workgroup (SPD: no pivoting).  The Schur complements must stay positive
+// definite: in FP32 their error, ~ eps cond(A11) |A21|^2, destroys that at
+// the library's conditioning, in FP64 it is ~1e-9 of the smallest eigenvalue.
 // ---------------------------------------------------------------------------
-constexpr int INV_BASE = 64;
+constexpr int INV_BASE = 128;
 
-// One wavefront, symmetric sweep operator: lane j holds column j of the
-// (nb <= 64) block in registers, read from the lower triangle and padded with
-// the identity.  Sweeping pivot k keeps the matrix symmetric, so the pivot
-// column is the pivot row, which every lane publishes with one LDS store:
-//   a_kk <- -1/a_kk,  a_ik <- a_ik/a_kk,  a_ij <- a_ij - a_ik a_kj / a_kk.
+// One workgroup of 256 threads, symmetric sweep operator on the (nb <= 128)
+// block, read from the lower triangle and padded with the identity.  Thread t
+// keeps the 8 x 8 block (rows 8 (t / 16) + r, columns 8 (t % 16) + c) in
+// registers.  Sweeping pivot k keeps the matrix symmetric, so pivot row and
+// pivot column are the same numbers: the threads holding column k publish it
+// to LDS, and every element updates from it:
+//   a_kk <- -1/a_kk,  a_ik <- a_ik/a_kk,  a_kj <- a_kj/a_kk,
+//   a_ij <- a_ij - a_ik a_kj / a_kk.
 // Sweeping all pivots leaves -A^-1; the full block is written back negated.
-// The column registers rotate by one per pivot (v[r] holds row (k + r) mod
-// 64 at step k), so every register index is static.
-__global__ __launch_bounds__(64) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
-  __shared__ double pr[INV_BASE];
-  const int j = threadIdx.x;
-  double v[INV_BASE];
+// Pivots run in groups of 8 with the position inside the group unrolled, so
+// every register index is static.
+__global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
+  __shared__ double col[INV_BASE];
+  const int t = threadIdx.x;
+  const int bi = t >> 4, bj = t & 15;
+  double a[8][8];
 #pragma unroll
-  for (int i = 0; i < INV_BASE; ++i) {
-    double x = (i == j) ? 1.0 : 0.0;
-    if (i < nb && j < nb) x = (i >= j) ? A[(int64_t)i * lda + j] : A[(int64_t)j * lda + i];
-    v[i] = x;
-  }
-  for (int k = 0; k < INV_BASE; ++k) {
-    pr[j] = v[0];                       // a_kj = a_jk
-    __syncthreads();
-    const double p = 1.0 / pr[k];
-    const double akj = v[0];
-    const bool pivot_col = (j == k);
+  for (int r = 0; r < 8; ++r)
 #pragma unroll
-    for (int r = 1; r < INV_BASE; ++r) {
-      const double aik = pr[(k + r) & (INV_BASE - 1)];
-      v[r] = pivot_col ? aik * p : v[r] - aik * akj * p;
+    for (int c = 0; c < 8; ++c) {
+      const int i = 8 * bi + r, j = 8 * bj + c;
+      double x = (i == j) ? 1.0 : 0.0;
+      if (i < nb && j < nb) x = (i >= j) ? A[(int64_t)i * lda + j] : A[(int64_t)j * lda + i];
+      a[r][c] = x;
     }
-    const double vk = pivot_col ? -p : akj * p;
+  for (int kb = 0; kb < INV_BASE / 8; ++kb) {
 #pragma unroll
-    for (int r = 0; r < INV_BASE - 1; ++r) v[r] = v[r + 1];
-    v[INV_BASE - 1] = vk;
-    __syncthreads();
+    for (int kc = 0; kc < 8; ++kc) {
+      const int k = 8 * kb + kc;
+      if (bj == kb) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) col[8 * bi + r] = a[r][kc];
+      }
+      __syncthreads();
+      const double f = 1.0 / col[k];
+      double gr[8], gc[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) gr[r] = col[8 * bi + r] * f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) gc[c] = col[8 * bj + c];
+      if (bi != kb && bj != kb) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) a[r][c] -= gr[r] * gc[c];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const bool rk = (bi == kb) && (r == kc), ck = (bj == kb) && (c == kc);
+            a[r][c] = rk ? (ck ? -f : gc[c] * f) : (ck ? gr[r] : a[r][c] - gr[r] * gc[c]);
+          }
+      }
+      __syncthreads();
+    }
   }
 #pragma unroll
-  for (int i = 0; i < INV_BASE; ++i)
-    if (i < nb && j < nb) A[(int64_t)i * lda + j] = -v[i];
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = 8 * bi + r, j = 8 * bj + c;
+      if (i < nb && j < nb) A[(int64_t)i * lda + j] = -a[r][c];
+    }
 }
 
 // Upper triangle of the n x n matrix <- its lower triangle, through 32 x 32
@@ -808,19 +825,15 @@ static void mirror_lower(hipStream_t st, T* A, int64_t n, int64_t lda) {
   hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(tiles), dim3(256), 0, st, A, n, lda, nt);
 }
 
-// Products in NT form (both operands k-contiguous):
-//   Tt = X A21^T,  T = Tt^T,  A22 -= T A21^T (lower),  A21 = -S^-1 Tt^T,
-//   A12 = A21^T,  A11 -= Tt A12^T (lower).  work holds 2 n1 n2 per level, n^2
-// in total suffices.
 // ---------------------------------------------------------------------------
 // The inverse's products are plain FP64 GEMMs, run by rocBLAS (its library
 // kernels reach 0.83-0.92 of the FP64 MFMA peak at n = 5k-10k here, against
-// about 0.5 for gemm_nt_kernel<double>).  Row-major X is the column-major X^T,
-// so C = alpha A B^T + beta C (row-major, A: M x K, B: N x K) is the
-// column-major C^T = alpha (B^T)^T A^T + beta C^T: dgemm(T, N, N, M, K) with
-// the operands swapped.  A product known to be symmetric updates only the
-// row-major lower triangle, the column-major upper one: dsyrkx(upper, T).
-// One handle per device, created on first use.
+// about 0.5 for gemm_nt_kernel<double>).  rocBLAS is column-major: a row-major
+// r x c array X with leading dimension ld is the column-major c x r matrix
+// X^T, so every product below is written for those transposed views.  A
+// product known to be symmetric updates only the row-major lower triangle,
+// i.e. the column-major upper one (dsyrkx).  One handle per device, created
+// on first use.
 // ---------------------------------------------------------------------------
 static rocblas_handle blas_handle(hipStream_t st) {
   static std::mutex mu;
@@ -836,54 +849,65 @@ static rocblas_handle blas_handle(hipStream_t st) {
   return handles[dev];
 }
 
-static int dgemm_nt_blas(hipStream_t st, int64_t M, int64_t N, int64_t K, double alpha,
-                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                         double* C, int64_t ldc, int symmetric_lower = 0) {
-  if (M <= 0 || N <= 0) return 0;
+// column-major C (m x n) = alpha op(P) op(Q) + beta C
+static int blas_dgemm(hipStream_t st, bool tP, bool tQ, int64_t m, int64_t n, int64_t k,
+                      double alpha, const double* P, int64_t ldp, const double* Q, int64_t ldq,
+                      double beta, double* C, int64_t ldc) {
+  if (m <= 0 || n <= 0) return 0;
   rocblas_handle h = blas_handle(st);
   SMX_REQUIRE(h, "rocBLAS handle could not be created");
-  rocblas_status r;
-  if (symmetric_lower)
-    r = rocblas_dsyrkx(h, rocblas_fill_upper, rocblas_operation_transpose, (rocblas_int)N,
-                       (rocblas_int)K, &alpha, B, (rocblas_int)ldb, A, (rocblas_int)lda, &beta, C,
-                       (rocblas_int)ldc);
-  else
-    r = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)N,
-                      (rocblas_int)M, (rocblas_int)K, &alpha, B, (rocblas_int)ldb, A,
-                      (rocblas_int)lda, &beta, C, (rocblas_int)ldc);
-  SMX_REQUIRE(r == rocblas_status_success, "rocBLAS FP64 product failed (status %d)", (int)r);
+  const rocblas_status r = rocblas_dgemm(
+      h, tP ? rocblas_operation_transpose : rocblas_operation_none,
+      tQ ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
+      (rocblas_int)k, &alpha, P, (rocblas_int)ldp, Q, (rocblas_int)ldq, &beta, C, (rocblas_int)ldc);
+  SMX_REQUIRE(r == rocblas_status_success, "rocblas_dgemm failed (status %d)", (int)r);
   return 0;
 }
 
+// column-major upper triangle of C (n x n) = alpha op(P) op(Q)^T + beta C,
+// op(P) op(Q)^T known to be symmetric
+static int blas_dsyrkx(hipStream_t st, bool trans, int64_t n, int64_t k, double alpha,
+                       const double* P, int64_t ldp, const double* Q, int64_t ldq, double beta,
+                       double* C, int64_t ldc) {
+  if (n <= 0) return 0;
+  rocblas_handle h = blas_handle(st);
+  SMX_REQUIRE(h, "rocBLAS handle could not be created");
+  const rocblas_status r = rocblas_dsyrkx(
+      h, rocblas_fill_upper, trans ? rocblas_operation_transpose : rocblas_operation_none,
+      (rocblas_int)n, (rocblas_int)k, &alpha, P, (rocblas_int)ldp, Q, (rocblas_int)ldq, &beta, C,
+      (rocblas_int)ldc);
+  SMX_REQUIRE(r == rocblas_status_success, "rocblas_dsyrkx failed (status %d)", (int)r);
+  return 0;
+}
+
+// work holds T (n2 x n1, row-major, leading dimension n1) per level; n^2 in
+// total suffices.  In the column-major views (A21^T is n1 x n2, T^T n1 x n2,
+// the diagonal blocks are symmetric):
+//   T^T = X A21^T                      dgemm(N, N, n1, n2, n1)
+//   S  -= T A21^T   (lower)            dsyrkx(T): (A21^T)^T T^T ... = A21 T^T
+//   A21^T = -T^T S^-1                  dgemm(N, N, n1, n2, n2)
+//   X  -= T^T A21   (lower)            dsyrkx(N): A21^T (T^T)^T = A21^T T
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
-    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(64), 0, st, A, lda, (int)n);
+    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
     SMX_LAUNCHED();
     return 0;
   }
   const int64_t n1 = ((n / 2 + INV_BASE - 1) / INV_BASE) * INV_BASE;
   const int64_t n2 = n - n1;
   double* A11 = A;
-  double* A12 = A + n1;
   double* A21 = A + n1 * lda;
   double* A22 = A + n1 * lda + n1;
-  double* Tt = work;                      // n1 x n2
-  double* Tm = work + n1 * n2;            // n2 x n1
-  double* deeper = work + 2 * n1 * n2;
+  double* T = work;                       // n2 x n1
+  double* deeper = work + n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
   mirror_lower(st, A11, n1, lda);
-  if (int e = dgemm_nt_blas(st, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, Tt, n2)) return e;
-  hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
-                     Tt, n2, n1, n2, Tm, n1);
-  SMX_LAUNCHED();
-  if (int e = dgemm_nt_blas(st, n2, n2, n1, -1.0, Tm, n1, A21, lda, 1.0, A22, lda, 1)) return e;
+  if (int e = blas_dgemm(st, false, false, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, T, n1)) return e;
+  if (int e = blas_dsyrkx(st, true, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda)) return e;
   if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
   mirror_lower(st, A22, n2, lda);
-  if (int e = dgemm_nt_blas(st, n2, n1, n2, -1.0, A22, lda, Tt, n2, 0.0, A21, lda)) return e;
-  hipLaunchKernelGGL(transpose_into_kernel<double>, dim3(blocks_for(n1 * n2, 256)), dim3(256), 0, st,
-                     A21, lda, n2, n1, A12, lda);
-  SMX_LAUNCHED();
-  return dgemm_nt_blas(st, n1, n1, n2, -1.0, Tt, n2, A12, lda, 1.0, A11, lda, 1);
+  if (int e = blas_dgemm(st, false, false, n1, n2, n2, -1.0, T, n1, A22, lda, 0.0, A21, lda)) return e;
+  return blas_dsyrkx(st, false, n1, n2, -1.0, A21, lda, T, n1, 1.0, A11, lda);
 }
 
 // Full inverse of the SPD matrix whose lower triangle is given.

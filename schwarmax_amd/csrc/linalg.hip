@@ -272,6 +272,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int AG_KR = 128;   // rows of R per workgroup
 constexpr int AG_BN = 256;   // columns of M per workgroup
+constexpr int AG_OCC = 2;    // workgroups per CU (LDS and registers)
+constexpr int AG_CB = AG_BN / 128;   // 32-column blocks per wave
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
 
 __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
@@ -380,7 +382,7 @@ struct AdmmStage {
 };
 
 template <int KC>
-__global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
+__global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
     int64_t Kp, int64_t npad, int chunks_per_split, const _Float16* __restrict__ Rh,
     const _Float16* __restrict__ Rl, const _Float16* __restrict__ Mh,
     const _Float16* __restrict__ Ml, float* __restrict__ part) {
@@ -402,11 +404,11 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
   const int64_t nch = npad / KC;
   const int64_t ch0 = (int64_t)split * chunks_per_split;
   const int64_t ch1 = ch0 + chunks_per_split < nch ? ch0 + chunks_per_split : nch;
-  f32x16 acc[4][2];
+  f32x16 acc[4][AG_CB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < AG_CB; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
   AdmmStage<KC> stg;
@@ -419,18 +421,18 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
     const bool more = ch + 1 < ch1;
 #pragma unroll
     for (int t = 0; t < KC / 16; ++t) {
-      half8 bh[2], bl[2];
+      half8 bh[AG_CB], bl[AG_CB];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
-        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
+      for (int cb = 0; cb < AG_CB; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
+        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
       }
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
         const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
         const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
+        for (int cb = 0; cb < AG_CB; ++cb) {
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
@@ -445,11 +447,11 @@ __global__ __launch_bounds__(256, 2) void admm_gemm_kernel(
     __syncthreads();
   }
   // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
-  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * 64;
+  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / 4);
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < AG_CB; ++cb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
@@ -658,10 +660,10 @@ struct NnlsWs {
 };
 
 // splits of the reduction so that the ADMM product fills the chip once
-// (two workgroups on each of the 256 CUs)
+// (AG_OCC workgroups on each of the 256 CUs)
 static int admm_splits(int64_t npad, int64_t Kp) {
   const int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);
-  int64_t S = 512 / (tiles > 0 ? tiles : 1);
+  int64_t S = (256 * AG_OCC) / (tiles > 0 ? tiles : 1);
   if (S < 1) S = 1;
   if (S > 16) S = 16;
   const int64_t nch = npad / AG_KP;

@@ -258,7 +258,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes': nbytes}))
         return
-    from schwarmax_amd import _lib, likelihood as L
+    from schwarmax_amd import _lib, likelihood as L, nnls as NN
     from oracle import opcount
     from oracle import model as OM
     if args.config.startswith('batched'):
@@ -323,7 +323,12 @@ def main():
     int_ms = stages['integrate'][0] / max(stages['integrate'][1], 1)
     int_flops = n_int * N * (3 * f_acc + opcount.F_STEP)
     prod_ms = stages['admm_product'][0] / max(stages['admm_product'][1], 1)
-    prod_bytes = 4.0 * n * n + 4.0 * K * n + 4.0 * K * n
+    # algorithmic bytes per launch: both fp16 planes of M (4 n^2) on a full
+    # product, the hi plane (2 n^2) on a delta product; the operand R or D in
+    # fp16 hi + lo (4 K n) and the FP32 product out (4 K n)
+    n_prod = max(NN.STATS['products'], 1)
+    full_frac = NN.STATS['full_products'] / n_prod
+    prod_bytes = (4.0 * full_frac + 2.0 * (1.0 - full_frac)) * n * n + 4.0 * K * n + 4.0 * K * n
     if stages['integrate'][0] >= stages['admm_product'][0]:
         ach = int_flops / (int_ms * 1e-3) / 1e12
         roof = {'bound': 'mfma', 'achieved': ach, 'peak': PEAK['fp64'], 'unit': 'TFLOP/s',
@@ -335,7 +340,8 @@ def main():
         roof = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK['hbm'], 'unit': 'GB/s',
                 'frac': ach / PEAK['hbm'], 'traffic': traffic_per_launch(args.config),
                 'kernel': 'admm_gemm_kernel', 'per_launch': prod_bytes,
-                'ms_per_launch': prod_ms, 'launches': stages['admm_product'][1]}
+                'ms_per_launch': prod_ms, 'launches': stages['admm_product'][1],
+                'full_products': NN.STATS['full_products'], 'products': NN.STATS['products']}
     out = {
         'metric': ('gradient evals/s @20k orbits' if grad_mode else
                    'model evals/s @20k orbits (plain)'), 'value': args.steps / dt,

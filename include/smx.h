@@ -240,6 +240,12 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
                   const double* row_scale, const double* Y, int64_t K,
                   double lambda_reg, int32_t maxiter, double w_init, double tol,
                   void* workspace, double* W, int32_t* iters_done, void* stream);
+/* How many of the last smx_nnls_admm's iterations on this workspace took the
+ * full ADMM product; the others took the delta product on the hi plane of
+ * the inverse (no reference counterpart: reported for the byte accounting of
+ * the product kernel).  Synchronises the stream. */
+int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* workspace,
+                                int32_t* full_products, void* stream);
 
 /* ---- dense building blocks (no reference counterpart: the reference calls
  * XLA's dot / cholesky / cho_solve; exposed for tests) ------------------- */

@@ -112,6 +112,7 @@ _SIGS = [
     ('smx_nnls_workspace_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_admm', ctypes.c_int,
      [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
+    ('smx_nnls_admm_full_products', ctypes.c_int, [_I64, _I64, _I64, _P, ctypes.POINTER(_I32), _P]),
     ('smx_gemm_nt_f32', ctypes.c_int, [_I64, _I64, _I64, ctypes.c_float, _P, _I64, _P, _I64,
                                        ctypes.c_float, _P, _I64, _P]),
     ('smx_gemm_nt_f64', ctypes.c_int, [_I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P, _I64, _P]),
@@ -186,8 +187,12 @@ STAGES = ('ics', 'integrate', 'finalize', 'gram', 'factor', 'rhs', 'admm', 'maps
           'admm_product', 'adjoint', 'jacobian')
 
 
+TIMING = {'on': False}
+
+
 def timing_enable(on=True):
     check(load_library().smx_timing_enable(1 if on else 0), 'smx_timing_enable')
+    TIMING['on'] = bool(on)
 
 
 def timing_read():

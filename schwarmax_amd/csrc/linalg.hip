@@ -240,17 +240,22 @@ __global__ void to_float_kernel(const double* __restrict__ src, int64_t rows, in
 // ---------------------------------------------------------------------------
 // ADMM helpers
 // ---------------------------------------------------------------------------
-__global__ void zmax_kernel(int K, int64_t n, const double* __restrict__ z, unsigned* __restrict__ zmax) {
+// The ADMM iterate is kept as one FP64 array x = w_hat + u_old: then
+// z = max(0, x) and u = min(0, x) exactly, as nnls.py:155-156 compute them
+// (z = max(0, x); u = (u_old + w_hat) - z is x - x = 0 or x - 0 = x).
+__global__ void zmax_kernel(int K, int64_t n, const double* __restrict__ x, unsigned* __restrict__ zmax) {
   const int k = blockIdx.y;
   float m = 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    m = fmaxf(m, static_cast<float>(fabs(z[(int64_t)k * n + i])));
+    m = fmaxf(m, static_cast<float>(fmax(0.0, x[(int64_t)k * n + i])));
   if (m > 0.f) atomicMax(zmax + k, __float_as_uint(m));
 }
 
-__global__ void copy_out_kernel(int64_t total, const double* __restrict__ z, double* __restrict__ W) {
+// W = z = max(0, x) after at least one iteration, z_init = w_init before any.
+__global__ void copy_out_kernel(int64_t total, const double* __restrict__ x, int iterated,
+                                double w_init, double* __restrict__ W) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < total) W[i] = z[i];
+  if (i < total) W[i] = iterated ? fmax(0.0, x[i]) : w_init;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,6 +266,18 @@ __global__ void copy_out_kernel(int64_t total, const double* __restrict__ z, dou
 // hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_f16 with FP32 accumulation.
 // Every fp16 product is exact in FP32, so the result carries FP32-level
 // error; the dropped lo*lo term is 2^-22 relative.
+//
+// Delta products.  R changes slowly from one iteration to the next (its rows
+// move by 1e-4 .. 1e-5 of their size per iteration at the library's
+// conditioning), so after a full product w_b = M R_b an iteration may take
+//   w = w_b + M_hi (R - R_b),
+// reading only the hi plane of M: half the bytes.  Its error,
+// M_lo (R - R_b) ~ 2^-12 |M| |R - R_b|, stays at the full product's 2^-22
+// |M| |R| while every row satisfies max|R_k - R_b,k| <= tau max|R_k| with
+// tau = 2^-10; an iteration where some row does not takes the full product
+// and makes its R the new base.  The choice is made on the device, from the
+// row maxima the previous update collected, identically in the product and
+// the update kernel, so the loop needs no host synchronisation.
 //
 // Work split: a workgroup (4 waves) owns 128 rows of R x 256 columns of M and
 // a contiguous range of the reduction index i; it writes its partial sum (no
@@ -275,6 +292,19 @@ constexpr int AG_BN = 256;   // columns of M per workgroup
 constexpr int AG_OCC = 2;    // workgroups per CU (LDS and registers)
 constexpr int AG_CB = AG_BN / 128;   // 32-column blocks per wave
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
+constexpr float AG_TAU = 1.0f / 1024.0f;   // delta-product bound max|R - R_b| / max|R| per row
+
+// Full product this iteration?  Forced, or some row k < K has max|D_k| >
+// tau max|R_k| (D = R - R_b).  Block-uniform; every thread must call it.
+__device__ __forceinline__ bool admm_full_product(int force, int64_t K,
+                                                  const unsigned* __restrict__ rmax,
+                                                  const unsigned* __restrict__ dmax) {
+  int need = force;
+  if (!need)
+    for (int64_t k = threadIdx.x; k < K; k += blockDim.x)
+      if (__uint_as_float(dmax[k]) > AG_TAU * __uint_as_float(rmax[k])) need = 1;
+  return __syncthreads_or(need) != 0;
+}
 
 __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
   // 2^(target - ceil(log2 amax)), or 1 for an all-zero row
@@ -288,12 +318,10 @@ __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
 // chunk) tile a workgroup loads is one contiguous run:
 //   M: tile (J, c) = M[256 J + r][32 c + q], r < 256, q < 32, at
 //      ((J * nch + c) * 256 + r) * 32 + q   (nch = npad / 32);
-//   R: tile c = R[k][32 c + q], k < Kp, at (c * Kp + k) * 32 + q.
+//   R (FP32, split into fp16 hi / lo as the product loads it): tile c =
+//      R[k][32 c + q], k < Kp, at (c * Kp + k) * 32 + q.
 __device__ __forceinline__ int64_t m_tile_index(int64_t j, int64_t i, int64_t nch) {
   return ((j / AG_BN * nch + i / AG_KP) * AG_BN + j % AG_BN) * AG_KP + i % AG_KP;
-}
-__device__ __forceinline__ int64_t r_tile_index(int64_t k, int64_t i, int64_t Kp) {
-  return ((i / AG_KP) * Kp + k) * AG_KP + i % AG_KP;
 }
 
 // M (n, n) FP64 -> scaled fp16 hi / lo (npad x npad, blocked), zero padded.
@@ -311,28 +339,13 @@ __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t 
   Ml[d] = static_cast<_Float16>(v - static_cast<double>(h));
 }
 
-// R (FP32, Kp x npad) -> scaled fp16 hi / lo (blocked) with each row's
-// power-of-two scale (max |R_k| -> 2^14).
-__global__ void split_r_kernel(const float* __restrict__ R, int64_t Kp, int64_t npad,
-                               const unsigned* __restrict__ rowmax, _Float16* __restrict__ Rh,
-                               _Float16* __restrict__ Rl) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= Kp * npad) return;
-  const int64_t k = idx / npad, i = idx - k * npad;
-  const float v = R[idx] * pow2_scale_for(__uint_as_float(rowmax[k]), 14);
-  const _Float16 h = static_cast<_Float16>(v);
-  const int64_t d = r_tile_index(k, i, Kp);
-  Rh[d] = h;
-  Rl[d] = static_cast<_Float16>(v - static_cast<float>(h));
-}
-
 // One chunk (KC reduction indices) of the ADMM product's operands.  B: 256
 // rows of M (hi and lo), thread t moves 16-byte segment t%TPR of row
 // t/TPR + (256/TPR)p, TPR = KC/8 lanes covering one row piece; A: 128 rows of
-// R (hi and lo), same pattern.  Loading the next chunk into registers during
-// the MFMAs (a 64-wide chunk at one workgroup per CU, or a 32-wide one) spills
-// and runs at half the speed of this load-after-compute form at two
-// workgroups per CU.
+// R, same pattern, split into hi and lo on the way to LDS.  The hi plane of
+// the next chunk is loaded into registers before the MFMAs of the current
+// one; the lo plane and A (from L2) after them.  Prefetching both planes
+// does not fit the registers of two workgroups per CU.
 template <int KC>
 struct AdmmStage {
   static_assert(KC == AG_KP, "the blocked split layouts use 32-wide chunks");
@@ -343,54 +356,126 @@ struct AdmmStage {
   uint4 bh[PB_], bl[PB_];
   // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
   // row t/4 + 64p, segment t%4, i.e. the tile's uint4 number t + 256p
-  __device__ void load_b(const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml,
-                         int64_t npad, int64_t j0, int64_t i0) {
+  // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
+  // row t/4 + 64p, segment t%4, i.e. the tile's uint4 number t + 256p.
+  // M streams once per launch: non-temporal loads keep it from evicting the
+  // R tiles that the workgroups of one split share in L2.
+  __device__ static void load_plane(const _Float16* __restrict__ M, int64_t npad, int64_t j0,
+                                    int64_t i0, uint4 (&dst)[PB_]) {
     const int t = threadIdx.x;
     const int64_t base = ((j0 / AG_BN) * (npad / AG_KP) + i0 / AG_KP) * (int64_t)(AG_BN * AG_KP);
-    // M streams once per launch: non-temporal loads keep it from evicting the
-    // R tiles that the workgroups of one split share in L2.
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
     for (int p = 0; p < PB_; ++p) {
       const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
-      const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Mh + off));
-      const u32x4 l = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Ml + off));
-      bh[p] = make_uint4(h.x, h.y, h.z, h.w);
-      bl[p] = make_uint4(l.x, l.y, l.z, l.w);
+      const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(M + off));
+      dst[p] = make_uint4(h.x, h.y, h.z, h.w);
     }
   }
-  template <int P>
+  // this thread's rows of R in the A tile (t/TPR + RPP p) and their scales:
+  // max |R_k| -> 2^14, a power of two, so hi + lo = R * scale exactly to
+  // 2^-22 relative
+  float rsc[PA_];
+  __device__ void row_scales(const unsigned* __restrict__ rowmax, int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < PA_; ++p)
+      rsc[p] = pow2_scale_for(__uint_as_float(rowmax[k0 + threadIdx.x / TPR + RPP * p]), 14);
+  }
+  template <bool FULL, int P>
   __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P], _Float16 (*Al)[P],
-                        const _Float16* __restrict__ Rh, const _Float16* __restrict__ Rl,
-                        int64_t Kp, int64_t k0, int64_t i0) const {
+                        const float* __restrict__ R, int64_t Kp, int64_t k0, int64_t i0) const {
     const int t = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PB_; ++p) {
       *reinterpret_cast<uint4*>(&Bh[t / TPR + RPP * p][8 * (t % TPR)]) = bh[p];
-      *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
+      if (FULL) *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
     }
     const int64_t base = ((i0 / AG_KP) * Kp + k0) * AG_KP;
 #pragma unroll
     for (int p = 0; p < PA_; ++p) {
       const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
-      *reinterpret_cast<uint4*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) =
-          *reinterpret_cast<const uint4*>(Rh + off);
-      *reinterpret_cast<uint4*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) =
-          *reinterpret_cast<const uint4*>(Rl + off);
+      const float4 v0 = *reinterpret_cast<const float4*>(R + off);
+      const float4 v1 = *reinterpret_cast<const float4*>(R + off + 4);
+      const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      half8 h, l;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float sv = v[e] * rsc[p];
+        h[e] = static_cast<_Float16>(sv);
+        l[e] = static_cast<_Float16>(sv - static_cast<float>(h[e]));
+      }
+      *reinterpret_cast<half8*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) = h;
+      *reinterpret_cast<half8*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) = l;
     }
   }
 };
 
+// Main loop over the split's chunks.  FULL: hi*hi + hi*lo + lo*hi of R and
+// M; otherwise hi*hi + lo*hi of the operand (D = R - R_b) and M's hi plane.
+template <int KC, bool FULL, int P>
+__device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t k0, int64_t j0,
+                                               int64_t ch0, int64_t ch1, const float* __restrict__ A,
+                                               const _Float16* __restrict__ Mh,
+                                               const _Float16* __restrict__ Ml, AdmmStage<KC>& stg,
+                                               _Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P],
+                                               _Float16 (*Al)[P], f32x16 (&acc)[4][AG_CB]) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 5, c = lane & 31;
+  if (ch0 < ch1) {
+    stg.load_plane(Mh, npad, j0, ch0 * KC, stg.bh);
+    if (FULL) stg.load_plane(Ml, npad, j0, ch0 * KC, stg.bl);
+    stg.template store<FULL>(Bh, Bl, Ah, Al, A, Kp, k0, ch0 * KC);
+  }
+  __syncthreads();
+  for (int64_t ch = ch0; ch < ch1; ++ch) {
+    const bool more = ch + 1 < ch1;
+    if (more) stg.load_plane(Mh, npad, j0, (ch + 1) * KC, stg.bh);
+#pragma unroll
+    for (int t = 0; t < KC / 16; ++t) {
+      half8 bh[AG_CB], bl[AG_CB];
+#pragma unroll
+      for (int cb = 0; cb < AG_CB; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
+        if (FULL)
+          bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
+      }
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
+        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
+#pragma unroll
+        for (int cb = 0; cb < AG_CB; ++cb) {
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+          if (FULL)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+    if (more) {
+      if (FULL) stg.load_plane(Ml, npad, j0, (ch + 1) * KC, stg.bl);
+      stg.template store<FULL>(Bh, Bl, Ah, Al, A, Kp, k0, (ch + 1) * KC);
+    }
+    __syncthreads();
+  }
+}
+
+// One ADMM product: the full one on R (scales from rmax), or the delta one on
+// D (scales from dmax), as admm_full_product decides.  D == nullptr: always
+// full (force must then be 1).
 template <int KC>
 __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
-    int64_t Kp, int64_t npad, int chunks_per_split, const _Float16* __restrict__ Rh,
-    const _Float16* __restrict__ Rl, const _Float16* __restrict__ Mh,
+    int64_t K, int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
+    const unsigned* __restrict__ rmax, const float* __restrict__ D,
+    const unsigned* __restrict__ dmax, int force, const _Float16* __restrict__ Mh,
     const _Float16* __restrict__ Ml, float* __restrict__ part) {
   constexpr int P = KC + 8;
   __shared__ __align__(16) _Float16 Bh[AG_BN][P];
   __shared__ __align__(16) _Float16 Bl[AG_BN][P];
   __shared__ __align__(16) _Float16 Ah[AG_KR][P];
   __shared__ __align__(16) _Float16 Al[AG_KR][P];
+  const bool full = admm_full_product(force, K, rmax, dmax);
   // 1-D grid, row block of R fastest (the Kp/128 row blocks that share a tile
   // of M run back to back and share it through the caches), then column
   // block, then split; with one row block this is column block, then split.
@@ -412,39 +497,12 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
   AdmmStage<KC> stg;
-  if (ch0 < ch1) {
-    stg.load_b(Mh, Ml, npad, j0, ch0 * KC);
-    stg.store(Bh, Bl, Ah, Al, Rh, Rl, Kp, k0, ch0 * KC);
-  }
-  __syncthreads();
-  for (int64_t ch = ch0; ch < ch1; ++ch) {
-    const bool more = ch + 1 < ch1;
-#pragma unroll
-    for (int t = 0; t < KC / 16; ++t) {
-      half8 bh[AG_CB], bl[AG_CB];
-#pragma unroll
-      for (int cb = 0; cb < AG_CB; ++cb) {
-        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
-        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
-      }
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb) {
-        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
-        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
-#pragma unroll
-        for (int cb = 0; cb < AG_CB; ++cb) {
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();
-    if (more) {
-      stg.load_b(Mh, Ml, npad, j0, (ch + 1) * KC);
-      stg.store(Bh, Bl, Ah, Al, Rh, Rl, Kp, k0, (ch + 1) * KC);
-    }
-    __syncthreads();
+  if (full) {
+    stg.row_scales(rmax, k0);
+    admm_gemm_loop<KC, true>(Kp, npad, k0, j0, ch0, ch1, R, Mh, Ml, stg, Bh, Bl, Ah, Al, acc);
+  } else {
+    stg.row_scales(dmax, k0);
+    admm_gemm_loop<KC, false>(Kp, npad, k0, j0, ch0, ch1, D, Mh, Ml, stg, Bh, Bl, Ah, Al, acc);
   }
   // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
   float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / 4);
@@ -461,13 +519,19 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
 
 __device__ __forceinline__ float block_max_256(float v, float* red);
 
-// init: z = w_init, u = 0, R = rho (z - u) - c, row maxima of |R|.  Grid
-// (ceil(n / 256), K): one row per blockIdx.y, one atomic per block.
-__global__ __launch_bounds__(256) void admm_init2_kernel(int64_t K, int64_t n, int64_t npad,
+// Index of R[k][i] in the blocked FP32 layout the product reads.
+__device__ __forceinline__ int64_t r_tile_index(int64_t k, int64_t i, int64_t Kp) {
+  return ((i / AG_KP) * Kp + k) * AG_KP + i % AG_KP;
+}
+
+// init (nnls.py:122-125, 149): z = w_init, u = 0, R = rho (z - u) - c and the
+// row maxima of |R|.  Grid (ceil(n / 256), K): one row per blockIdx.y, one
+// atomic per block.  x is not written: the first update takes (z, u) =
+// (w_init, 0) from its arguments.
+__global__ __launch_bounds__(256) void admm_init2_kernel(int64_t K, int64_t n, int64_t Kp,
                                                          double w_init,
                                                          const double* __restrict__ rho_p,
                                                          const float* __restrict__ c,
-                                                         double* __restrict__ z, double* __restrict__ u,
                                                          float* __restrict__ R,
                                                          unsigned* __restrict__ rowmax) {
   __shared__ float red[4];
@@ -475,11 +539,8 @@ __global__ __launch_bounds__(256) void admm_init2_kernel(int64_t K, int64_t n, i
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   float m = 0.f;
   if (i < n) {
-    const int64_t idx = k * n + i;
-    z[idx] = w_init;
-    u[idx] = 0.0;
-    const float r = static_cast<float>(*rho_p * (w_init - 0.0) - static_cast<double>(c[idx]));
-    R[k * npad + i] = r;
+    const float r = static_cast<float>(*rho_p * (w_init - 0.0) - static_cast<double>(c[k * n + i]));
+    R[r_tile_index(k, i, Kp)] = r;
     m = fabsf(r);
   }
   m = block_max_256(m, red);
@@ -499,92 +560,201 @@ __device__ __forceinline__ float block_max_256(float v, float* red) {
 // ADMM step on the summed partial products (nnls.py:152-157), next R and its
 // row maxima.  Grid (ceil(n / (256 V)), K): one row per blockIdx.y, V
 // consecutive elements per thread (V = 4 needs n % 4 == 0, so that every
-// vector access is aligned), one atomic per block.  rowmax_cur scales this
-// iteration's product, rowmax_next collects the next one, rowmax_clear is
-// reset for the iteration after.
+// vector access is aligned; 4 consecutive indices stay inside one 32-wide
+// chunk of R's blocked layout), one atomic per block and maximum.
+//   full product:  w = sum / (sM sR); w_b <- w, R_b <- this iteration's R;
+//   delta product: w = w_b + sum / (sM sD).
+// It writes the next R and D = R - R_b for the next product, whichever kind
+// that is.  rmax/dmax: cur scales this iteration's product, next collects the
+// next one, clear is reset for the iteration after.  first != 0: the iterate
+// before this step is the initial (z, u) = (w_init, 0) rather than x.
 struct AdmmStep {
-  double rho, alpha, inv_scale;
-  __device__ __forceinline__ void apply(float sum, double zo, double uo, float cv, double& zn,
-                                        double& un, float& r) const {
-    const double w = static_cast<double>(sum) * inv_scale;
+  double rho, alpha;
+  // x <- w_hat + u_old with w_hat = alpha w + (1 - alpha) z_old; returns
+  // R = rho (z - u) - c of the new iterate and the step's |z - z_old|
+  __device__ __forceinline__ float apply(double w, double zo, double uo, float cv, double& x,
+                                         float& dz) const {
     const double wh = alpha * w + (1.0 - alpha) * zo;
-    zn = fmax(0.0, wh + uo);
-    un = (uo + wh) - zn;
-    r = static_cast<float>(rho * (zn - un) - static_cast<double>(cv));
+    x = wh + uo;
+    const double zn = fmax(0.0, x), un = fmin(0.0, x);
+    dz = static_cast<float>(fabs(zn - zo));
+    return static_cast<float>(rho * (zn - un) - static_cast<double>(cv));
   }
+};
+
+// Sum over the S split partials at one position, four loads in flight, added
+// in split order.
+template <int V>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&v)[V]) {
+  if constexpr (V == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = p[e];
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&v)[V]) {
+  if constexpr (V == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) p[e] = v[e];
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void load_vec(const double* __restrict__ p, double (&v)[V]) {
+  if constexpr (V == 4) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 b = *reinterpret_cast<const double2*>(p + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = p[e];
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void store_vec(double* __restrict__ p, const double (&v)[V]) {
+  if constexpr (V == 4) {
+    *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) p[e] = v[e];
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void sum_partials(const float* __restrict__ p, int64_t stride, int S,
+                                             float (&sum)[V]) {
+#pragma unroll
+  for (int e = 0; e < V; ++e) sum[e] = 0.f;
+  int s = 0;
+  for (; s + 4 <= S; s += 4) {
+    float a[4][V];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load_vec<V>(p + (int64_t)(s + q) * stride, a[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < V; ++e) sum[e] += a[q][e];
+  }
+  for (; s < S; ++s) {
+    float a[V];
+    load_vec<V>(p + (int64_t)s * stride, a);
+#pragma unroll
+    for (int e = 0; e < V; ++e) sum[e] += a[e];
+  }
+}
+
+// Per-iteration buffers of the ADMM loop (all device pointers).
+struct AdmmIter {
+  float* R;                 // Kp x npad blocked: this iteration's R, replaced by the next
+  float* Rb;                // Kp x npad blocked: R at the last full product
+  float* D;                 // Kp x npad blocked: next R - R_b
+  double* wb;               // K x n: w of the last full product
+  const unsigned* rcur;     // max |R_k| of this iteration's R
+  unsigned* rnext;
+  unsigned* rclear;
+  const unsigned* dcur;     // max |D_k| of this iteration's D
+  unsigned* dnext;
+  unsigned* dclear;
+  unsigned* full_count;     // full products taken
 };
 
 template <int V>
 __global__ __launch_bounds__(256) void admm_update_kernel(
     int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
-    const double* __restrict__ rho_p, const float* __restrict__ c, double* __restrict__ z,
-    double* __restrict__ u, float* __restrict__ R, const unsigned* __restrict__ rowmax_cur,
-    unsigned* __restrict__ rowmax_next, unsigned* __restrict__ rowmax_clear, double alpha,
-    const int* __restrict__ active, unsigned* __restrict__ delta) {
+    const double* __restrict__ rho_p, const float* __restrict__ c, double* __restrict__ x,
+    int first, int force, double w_init, AdmmIter b, double alpha, const int* __restrict__ active,
+    unsigned* __restrict__ delta) {
   __shared__ float red[4];
+  const bool full = admm_full_product(force, K, b.rcur, b.dcur);
   const int64_t k = blockIdx.y;
   const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * V;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    rowmax_clear[k] = 0u;
+    b.rclear[k] = 0u;
+    b.dclear[k] = 0u;
     if (k == K - 1)
-      for (int64_t q = K; q < Kp; ++q) rowmax_clear[q] = 0u;
-  }
-  const int64_t o = k * npad + i0;
-  float rmax = 0.f, dmax = 0.f;
-  if (!active[k]) {
-#pragma unroll
-    for (int e = 0; e < V; ++e)
-      if (i0 + e < n) rmax = fmaxf(rmax, fabsf(R[o + e]));
-  } else if (i0 < n) {
-    const int64_t idx = k * n + i0;
-    const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
-    const float sR = pow2_scale_for(__uint_as_float(rowmax_cur[k]), 14);
-    const AdmmStep st{*rho_p, alpha, 1.0 / (static_cast<double>(sM) * static_cast<double>(sR))};
-    if (V == 4) {
-      float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int s = 0; s < S; ++s) {
-        const float4 p4 = *reinterpret_cast<const float4*>(part + ((int64_t)s * Kp + k) * npad + i0);
-        sum.x += p4.x; sum.y += p4.y; sum.z += p4.z; sum.w += p4.w;
+      for (int64_t q = K; q < Kp; ++q) {
+        b.rclear[q] = 0u;
+        b.dclear[q] = 0u;
       }
-      const double2 z01 = *reinterpret_cast<const double2*>(z + idx);
-      const double2 z23 = *reinterpret_cast<const double2*>(z + idx + 2);
-      const double2 u01 = *reinterpret_cast<const double2*>(u + idx);
-      const double2 u23 = *reinterpret_cast<const double2*>(u + idx + 2);
-      const float4 c4 = *reinterpret_cast<const float4*>(c + idx);
-      double zn[4], un[4];
-      float r[4];
-      st.apply(sum.x, z01.x, u01.x, c4.x, zn[0], un[0], r[0]);
-      st.apply(sum.y, z01.y, u01.y, c4.y, zn[1], un[1], r[1]);
-      st.apply(sum.z, z23.x, u23.x, c4.z, zn[2], un[2], r[2]);
-      st.apply(sum.w, z23.y, u23.y, c4.w, zn[3], un[3], r[3]);
-      *reinterpret_cast<double2*>(z + idx) = make_double2(zn[0], zn[1]);
-      *reinterpret_cast<double2*>(z + idx + 2) = make_double2(zn[2], zn[3]);
-      *reinterpret_cast<double2*>(u + idx) = make_double2(un[0], un[1]);
-      *reinterpret_cast<double2*>(u + idx + 2) = make_double2(un[2], un[3]);
-      *reinterpret_cast<float4*>(R + o) = make_float4(r[0], r[1], r[2], r[3]);
-      rmax = fmaxf(fmaxf(fabsf(r[0]), fabsf(r[1])), fmaxf(fabsf(r[2]), fabsf(r[3])));
-      dmax = static_cast<float>(fmax(fmax(fabs(zn[0] - z01.x), fabs(zn[1] - z01.y)),
-                                     fmax(fabs(zn[2] - z23.x), fabs(zn[3] - z23.y))));
+    if (k == 0 && full) *b.full_count += 1u;
+  }
+  float rmax = 0.f, dmx = 0.f, zmv = 0.f;
+  if (i0 < n) {
+    const int64_t ri = r_tile_index(k, i0, Kp);
+    float rcur[V], rb[V], rn[V], dn[V];
+    load_vec<V>(b.R + ri, rcur);
+    if (full) {
+      store_vec<V>(b.Rb + ri, rcur);
+#pragma unroll
+      for (int e = 0; e < V; ++e) rb[e] = rcur[e];
     } else {
-      float sum = 0.f;
-      for (int s = 0; s < S; ++s) sum += part[((int64_t)s * Kp + k) * npad + i0];
-      const double zo = z[idx], uo = u[idx];
-      double zn, un;
-      float r;
-      st.apply(sum, zo, uo, c[idx], zn, un, r);
-      z[idx] = zn;
-      u[idx] = un;
-      R[o] = r;
-      rmax = fabsf(r);
-      dmax = static_cast<float>(fabs(zn - zo));
+      load_vec<V>(b.Rb + ri, rb);
     }
+    if (!active[k]) {
+#pragma unroll
+      for (int e = 0; e < V; ++e) rn[e] = rcur[e];
+    } else {
+      const int64_t idx = k * n + i0;
+      const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
+      const float sA = pow2_scale_for(__uint_as_float(full ? b.rcur[k] : b.dcur[k]), 14);
+      const double inv = 1.0 / (static_cast<double>(sM) * static_cast<double>(sA));
+      const AdmmStep st{*rho_p, alpha};
+      float sum[V];
+      sum_partials<V>(part + k * npad + i0, Kp * npad, S, sum);
+      double w[V], xo[V], xn[V];
+      float cv[V];
+      load_vec<V>(c + idx, cv);
+      if (full) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) w[e] = static_cast<double>(sum[e]) * inv;
+        store_vec<V>(b.wb + idx, w);
+      } else {
+        load_vec<V>(b.wb + idx, w);
+#pragma unroll
+        for (int e = 0; e < V; ++e) w[e] += static_cast<double>(sum[e]) * inv;
+      }
+      if (first) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) xo[e] = w_init;
+      } else {
+        load_vec<V>(x + idx, xo);
+      }
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const double zo = first ? w_init : fmax(0.0, xo[e]);
+        const double uo = first ? 0.0 : fmin(0.0, xo[e]);
+        float dz;
+        rn[e] = st.apply(w[e], zo, uo, cv[e], xn[e], dz);
+        zmv = fmaxf(zmv, dz);
+      }
+      store_vec<V>(x + idx, xn);
+      store_vec<V>(b.R + ri, rn);
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      dn[e] = rn[e] - rb[e];
+      rmax = fmaxf(rmax, fabsf(rn[e]));
+      dmx = fmaxf(dmx, fabsf(dn[e]));
+    }
+    store_vec<V>(b.D + ri, dn);
   }
   rmax = block_max_256(rmax, red);
   __syncthreads();
-  dmax = block_max_256(dmax, red);
+  dmx = block_max_256(dmx, red);
+  __syncthreads();
+  zmv = block_max_256(zmv, red);
   if (threadIdx.x == 0) {
-    if (rmax > 0.f) atomicMax(rowmax_next + k, __float_as_uint(rmax));
-    if (dmax > 0.f) atomicMax(delta + k, __float_as_uint(dmax));
+    if (rmax > 0.f) atomicMax(b.rnext + k, __float_as_uint(rmax));
+    if (dmx > 0.f) atomicMax(b.dnext + k, __float_as_uint(dmx));
+    if (zmv > 0.f) atomicMax(delta + k, __float_as_uint(zmv));
   }
 }
 
@@ -640,8 +810,7 @@ struct NnlsWs {
   double* work64; // n*n + n work of the recursive inverse
   float* Yf;      // K x ldu
   float* C;       // K x n
-  double* z;      // K x n
-  double* u;      // K x n
+  double* x;      // K x n ADMM iterate, z = max(0, x), u = min(0, x)
   float* rho;     // 1
   double* rho64;  // 1
   unsigned* delta;  // K
@@ -649,11 +818,13 @@ struct NnlsWs {
   int* active;      // K
   _Float16* Mh;     // npad x npad
   _Float16* Ml;     // npad x npad
-  float* Rp;        // Kp x npad
-  _Float16* Rh;     // Kp x npad
-  _Float16* Rl;     // Kp x npad
+  float* Rp;        // Kp x npad, blocked (r_tile_index)
+  float* Rb;        // Kp x npad, blocked: R at the last full product
+  float* D;         // Kp x npad, blocked: R - Rb
+  double* wb;       // K x n: w of the last full product
+  unsigned* full_count;  // full ADMM products taken by the last solve
   float* part;      // S x Kp x npad
-  unsigned* rmax;   // 3 x Kp
+  unsigned* rmax;   // 6 x Kp: three rotating row-maximum sets of |R|, then three of |D|
   int64_t npad, Kp;
   int S;
   int64_t bytes;
@@ -687,8 +858,7 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.work64 = reinterpret_cast<double*>(take((n * n + n) * 8));
   w.Yf = reinterpret_cast<float*>(take(K * ldu * 4));
   w.C = reinterpret_cast<float*>(take(K * n * 4));
-  w.z = reinterpret_cast<double*>(take(K * n * 8));
-  w.u = reinterpret_cast<double*>(take(K * n * 8));
+  w.x = reinterpret_cast<double*>(take(K * n * 8));
   w.rho = reinterpret_cast<float*>(take(4));
   w.rho64 = reinterpret_cast<double*>(take(8));
   w.delta = reinterpret_cast<unsigned*>(take(K * 4));
@@ -700,10 +870,12 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Mh = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Ml = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Rp = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
-  w.Rh = reinterpret_cast<_Float16*>(take(w.Kp * w.npad * 2));
-  w.Rl = reinterpret_cast<_Float16*>(take(w.Kp * w.npad * 2));
   w.part = reinterpret_cast<float*>(take((int64_t)w.S * w.Kp * w.npad * 4));
-  w.rmax = reinterpret_cast<unsigned*>(take(3 * w.Kp * 4));
+  w.rmax = reinterpret_cast<unsigned*>(take(6 * w.Kp * 4));
+  w.Rb = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
+  w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
+  w.wb = reinterpret_cast<double*>(take(K * n * 8));
+  w.full_count = reinterpret_cast<unsigned*>(take(4));
   w.bytes = off;
   return w;
 }
@@ -1002,13 +1174,15 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad, 256)), dim3(256), 0, st, w.Q64, n,
                      npad, w.rho, w.Mh, w.Ml);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st,
-                     reinterpret_cast<unsigned*>(w.Rp), Kp * npad, 0u);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(3 * Kp, 256)), dim3(256), 0, st, w.rmax,
-                     3 * Kp, 0u);
+  for (float* buf : {w.Rp, w.Rb, w.D})
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned*>(buf), Kp * npad, 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(6 * Kp, 256)), dim3(256), 0, st, w.rmax,
+                     6 * Kp, 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, w.full_count, 1, 0u);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(admm_init2_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st, K,
-                     n, npad, w_init, w.rho64, w.C, w.z, w.u, w.Rp, w.rmax);
+                     n, Kp, w_init, w.rho64, w.C, w.Rp, w.rmax);
   SMX_LAUNCHED();
   stage_end(ST_RHS, st, 5);
   std::vector<int> act(K, 1);
@@ -1024,29 +1198,40 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     const bool chk = tol > 0.0 && ((it + 1) % check == 0);
     if (chk)
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.delta, K, 0u);
-    unsigned* rcur = w.rmax + (it % 3) * Kp;
-    unsigned* rnext = w.rmax + ((it + 1) % 3) * Kp;
-    unsigned* rclear = w.rmax + ((it + 2) % 3) * Kp;
-    hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
-                       Kp, npad, rcur, w.Rh, w.Rl);
+    AdmmIter ib;
+    ib.R = w.Rp;
+    ib.Rb = w.Rb;
+    ib.D = w.D;
+    ib.wb = w.wb;
+    ib.rcur = w.rmax + (it % 3) * Kp;
+    ib.rnext = w.rmax + ((it + 1) % 3) * Kp;
+    ib.rclear = w.rmax + ((it + 2) % 3) * Kp;
+    ib.dcur = w.rmax + (3 + it % 3) * Kp;
+    ib.dnext = w.rmax + (3 + (it + 1) % 3) * Kp;
+    ib.dclear = w.rmax + (3 + (it + 2) % 3) * Kp;
+    ib.full_count = w.full_count;
+    // the first product has no base; the last is full so that the result
+    // carries no delta error
+    const int force = (it == 0 || it == maxiter - 1) ? 1 : 0;
+    const int first = it == 0 ? 1 : 0;
     stage_begin(ST_ADMM_PRODUCT, st);
-    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rh,
-                       w.Rl, w.Mh, w.Ml, w.part);
+    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
+                       ib.rcur, w.D, ib.dcur, force, w.Mh, w.Ml, w.part);
     SMX_LAUNCHED();
     stage_end(ST_ADMM_PRODUCT, st, 1);
     if (n % 4 == 0)
       hipLaunchKernelGGL(admm_update_kernel<4>, dim3(blocks_for(n / 4, 256), (unsigned)K), dim3(256),
-                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext,
-                         rclear, 1.6, w.active, w.delta);
+                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.x, first, force, w_init,
+                         ib, 1.6, w.active, w.delta);
     else
       hipLaunchKernelGGL(admm_update_kernel<1>, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0,
-                         st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.z, w.u, w.Rp, rcur, rnext,
-                         rclear, 1.6, w.active, w.delta);
+                         st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.x, first, force, w_init, ib,
+                         1.6, w.active, w.delta);
     SMX_LAUNCHED();
     done = it + 1;
     if (chk) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.zmax, K, 0u);
-      hipLaunchKernelGGL(zmax_kernel, dim3(64, (unsigned)K), dim3(256), 0, st, (int)K, n, w.z, w.zmax);
+      hipLaunchKernelGGL(zmax_kernel, dim3(64, (unsigned)K), dim3(256), 0, st, (int)K, n, w.x, w.zmax);
       SMX_HIP(hipMemcpyAsync(dl.data(), w.delta, K * 4, hipMemcpyDeviceToHost, st));
       SMX_HIP(hipMemcpyAsync(zm.data(), w.zmax, K * 4, hipMemcpyDeviceToHost, st));
       SMX_HIP(hipStreamSynchronize(st));
@@ -1063,7 +1248,8 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     }
   }
   stage_end(ST_ADMM, st, done);
-  hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w.z, W);
+  hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w.x,
+                     done > 0 ? 1 : 0, w_init, W);
   SMX_LAUNCHED();
   if (iters_done) *iters_done = done;
   return 0;
@@ -1076,8 +1262,10 @@ namespace smx {
 // then applied to any K right-hand sides (K <= the K the workspace was sized
 // for).  Used where many solves share one Gram up to low-rank changes.
 // ---------------------------------------------------------------------------
+// R (K, n) -> zero-padded (Kp, npad) in the product's blocked layout, and
+// the row maxima of |R|.  Grid (npad / 256, Kp).
 __global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__ R, int64_t K,
-                                                       int64_t n, int64_t npad,
+                                                       int64_t n, int64_t Kp,
                                                        float* __restrict__ Rp,
                                                        unsigned* __restrict__ rmax) {
   __shared__ float red[4];
@@ -1085,7 +1273,7 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   float v = 0.f;
   if (k < K && i < n) v = R[k * n + i];
-  if (i < npad) Rp[k * npad + i] = v;
+  Rp[r_tile_index(k, i, Kp)] = v;
   const float m = block_max_256(fabsf(v), red);
   if (threadIdx.x == 0 && m > 0.f) atomicMax(rmax + k, __float_as_uint(m));
 }
@@ -1132,6 +1320,19 @@ static int nnls_build_operator(hipStream_t st, const NnlsWs& w, const double* li
 using namespace smx;
 extern "C" {
 
+int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* workspace,
+                                int32_t* full_products, void* stream) {
+  SMX_REQUIRE(workspace && full_products && n > 0 && m > 0 && K > 0,
+              "smx_nnls_admm_full_products: bad arguments");
+  hipStream_t st = as_stream(stream);
+  NnlsWs w = carve(const_cast<void*>(workspace), n, m, K);
+  unsigned v = 0;
+  SMX_HIP(hipMemcpyAsync(&v, w.full_count, 4, hipMemcpyDeviceToHost, st));
+  SMX_HIP(hipStreamSynchronize(st));
+  *full_products = static_cast<int32_t>(v);
+  return 0;
+}
+
 int64_t smx_nnls_operator_bytes(int64_t n, int64_t m, int64_t K_max) {
   if (n <= 0 || m <= 0 || K_max <= 0) return -1;
   return carve(nullptr, n, m, K_max).bytes;
@@ -1162,15 +1363,13 @@ int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace
   const int64_t nch = npad / AG_KP;
   const int cps = static_cast<int>((nch + S - 1) / S);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp, 256)), dim3(256), 0, st, w.rmax, Kp, 0u);
-  hipLaunchKernelGGL(pad_rows_kernel, dim3(blocks_for(npad, 256), (unsigned)Kp), dim3(256), 0, st, R,
-                     K, n, npad, w.Rp, w.rmax);
-  hipLaunchKernelGGL(split_r_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st, w.Rp,
-                     Kp, npad, w.rmax, w.Rh, w.Rl);
+  hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)(npad / 256), (unsigned)Kp), dim3(256), 0, st,
+                     R, K, n, Kp, w.Rp, w.rmax);
   SMX_LAUNCHED();
   dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM_PRODUCT, st);
-  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, Kp, npad, cps, w.Rh, w.Rl,
-                     w.Mh, w.Ml, w.part);
+  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
+                     w.rmax, (const float*)nullptr, (const unsigned*)nullptr, 1, w.Mh, w.Ml, w.part);
   SMX_LAUNCHED();
   stage_end(ST_ADMM_PRODUCT, st, 1);
   hipLaunchKernelGGL(apply_out_kernel, dim3(blocks_for(K * n, 256)), dim3(256), 0, st, w.part, S,

@@ -87,7 +87,10 @@ __global__ __launch_bounds__(256) void initial_conditions_kernel(
 // lane, samples accumulated into the lane's own slab.  With ckpt, the state
 // (y, dt, t) before every ckpt_every-th attempt is stored for the adjoint.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void integrate_library_kernel(
+#ifndef SMX_INTEGRATOR_WAVES
+#define SMX_INTEGRATOR_WAVES 1   // waves per SIMD the integrator's registers are sized for
+#endif
+__global__ __launch_bounds__(64, SMX_INTEGRATOR_WAVES) void integrate_library_kernel(
     smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
     const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
     const int32_t* __restrict__ bm, const double* __restrict__ v0, const double* __restrict__ sd,

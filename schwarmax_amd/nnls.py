@@ -1,8 +1,10 @@
 """ADMM NNLS drop-ins (nnls.py:6-166) backed by smx_nnls_admm.
 
-The GPU solver forms U^T U in FP32, factors Q + rho I, inverts it once and
-runs each ADMM iteration as one FP32 product with the projection / dual update
-fused in (z and u kept in FP64).  Same iterate definition as the reference:
+The GPU solver forms U^T U in FP32, inverts Q + rho I once in FP64 and runs
+each ADMM iteration as one product with FP32-level accuracy (fp16 hi/lo splits
+on MFMA; while R moves little, the product of the inverse's hi plane with the
+change of R since the last full product) followed by the projection / dual
+update (iterates in FP64).  Same iterate definition as the reference:
 alpha = 1.6, rho = tr(Q)/n, start w = z = sum(y_Rzphi)/n, u = 0, ``maxiter``
 iterations, result z.  ``tol > 0`` lets a right-hand side stop once its iterate
 moves by less than tol * max|z| over 25 iterations.
@@ -14,6 +16,10 @@ import torch
 from . import _lib
 
 EPS = 1e-8
+
+# While stage timing is on: ADMM products taken and how many were full (the
+# others multiply the inverse's hi plane by R - R_base).
+STATS = {'products': 0, 'full_products': 0}
 
 
 def gamma_safe(gamma_kin):
@@ -44,6 +50,12 @@ def admm_solve(lib, m, row_scale, Y, lambda_reg, maxiter, w_init, tol=0.0):
                                _lib.ptr(Y.contiguous()), K, float(lambda_reg), int(maxiter),
                                float(w_init), float(tol), _lib.ptr(ws), _lib.ptr(W),
                                ctypes.byref(iters), _lib.stream_ptr()), 'smx_nnls_admm')
+    if _lib.TIMING['on']:
+        full = ctypes.c_int32(0)
+        _lib.check(L.smx_nnls_admm_full_products(n, m, K, _lib.ptr(ws), ctypes.byref(full),
+                                                 _lib.stream_ptr()), 'smx_nnls_admm_full_products')
+        STATS['products'] += int(iters.value)
+        STATS['full_products'] += int(full.value)
     return W, int(iters.value)
 
 

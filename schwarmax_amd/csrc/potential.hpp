@@ -94,10 +94,18 @@ __device__ __forceinline__ void acrn_fold(const S& xp, const S& u2, const S& iu2
     if (k > 0) {
       const double cur = 2.0 * k - 1.0;
       const S num = (cur - 1.0) * a + xp * ax;
-      const S an = EXACT ? num / (cur * u2) : num * (iu2 / cur);
-      b = (-ax) / cur;
+      if (EXACT) {
+        a = num / (cur * u2);
+        b = (-ax) / cur;
+      } else {
+        // force path: divisions by the unrolled order's constant become
+        // products with its reciprocal (one rounding more, no FP64 divide)
+        constexpr double one = 1.0;
+        const double icur = one / cur;
+        a = num * (iu2 * icur);
+        b = (-ax) * icur;
+      }
       ax = ax * iR2;
-      a = an;
     }
     if (sC == 0.0) {
       F[k] += sA * a;
@@ -230,12 +238,15 @@ __device__ __forceinline__ void nfw_grad_t(const smx_potential& p, const S& mGM,
   const S rx = (R[0] * dx + R[1] * dy) + R[2] * dz;
   const S ry = (R[3] * dx + R[4] * dy) + R[5] * dz;
   const S rz = (R[6] * dx + R[7] * dy) + R[8] * dz;
-  const S qx = rx / p.nfw_a, qy = ry / p.nfw_b, qz = rz / p.nfw_c;
+  // products with the axis reciprocals (invariant across calls) instead of
+  // FP64 divides; one rounding more than the quotients
+  const double ia = 1.0 / p.nfw_a, ib = 1.0 / p.nfw_b, ic = 1.0 / p.nfw_c;
+  const S qx = rx * ia, qy = ry * ib, qz = rz * ic;
   const S r = sqrt(((qx * qx + qy * qy) + qz * qz) + kEps);
-  const S re = r + kEps;
-  const S dPhi_dr = mGM * ((1.0 / (Rs + r)) / re - log(1.0 + r / Rs) / (re * re));
+  const S ire = 1.0 / (r + kEps);
+  const S dPhi_dr = mGM * ((1.0 / (Rs + r)) * ire - log(1.0 + r / Rs) * (ire * ire));
   const S f = dPhi_dr / r;
-  const S gqx = f * qx / p.nfw_a, gqy = f * qy / p.nfw_b, gqz = f * qz / p.nfw_c;
+  const S gqx = (f * qx) * ia, gqy = (f * qy) * ib, gqz = (f * qz) * ic;
   gx = (R[0] * gqx + R[3] * gqy) + R[6] * gqz;
   gy = (R[1] * gqx + R[4] * gqy) + R[7] * gqz;
   gz = (R[2] * gqx + R[5] * gqy) + R[8] * gqz;

@@ -47,3 +47,39 @@ def test_bootstrap_weights(maxiter):
     f_ref = np.sum((ref @ U.T - Y) ** 2, axis=1)
     f_got = np.sum((got @ U.T - Y) ** 2, axis=1)
     assert np.max(np.abs(f_got - f_ref) / f_ref) < 1e-4
+
+
+def test_delta_products_exercised():
+    """The solve mixes full products with delta products (the inverse's hi
+    plane times R - R_base) and still meets the oracle's bars; the first
+    and the last iteration are full."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from schwarmax_amd import nnls as GN, _lib
+    from oracle import nnls as ON
+    A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy, sh = _problem(2, n=300, K=12)
+    w5 = np.sqrt(5.0)
+    maxiter = 600
+    before = dict(GN.STATS)
+    _lib.timing_enable(True)
+    try:
+        got = GN.solve_nnls_admm_bootstrap(A_R, A_xy, *A_h, y_R, y_xy, gamma, yb_xy, *yb_h, sR, sxy,
+                                           *sh, lambda_reg=1.0, maxiter=maxiter, w_rzphi=w5,
+                                           w_xy=w5, w_h=1.0).cpu().numpy()
+    finally:
+        _lib.timing_enable(False)
+    products = GN.STATS['products'] - before['products']
+    full = GN.STATS['full_products'] - before['full_products']
+    print(f"ADMM products {products}, full {full}")
+    assert products == maxiter
+    assert 2 <= full < maxiter // 2, full
+    ref = ON.solve_nnls_admm_bootstrap(A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy, sh,
+                                       lambda_reg=1.0, maxiter=maxiter, w_rzphi=w5, w_xy=w5, w_h=1.0)
+    err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() < 1e-3, err
+    U = ON.design_matrix(A_R, A_xy, A_h, sR, sxy, sh, gamma, w5, w5, 1.0)
+    Y = ON.rhs_vectors(y_R, sR, yb_xy, yb_h, sxy, sh, w5, w5, 1.0)
+    f_ref = np.sum((ref @ U.T - Y) ** 2, axis=1)
+    f_got = np.sum((got @ U.T - Y) ** 2, axis=1)
+    assert np.max(np.abs(f_got - f_ref) / f_ref) < 1e-4

@@ -84,9 +84,10 @@ template <int NO, bool EXACT, bool GRAD, class S>
 __device__ __forceinline__ void acrn_fold(const S& xp, const S& u2, const S& iu2, double sA, double sC,
                                           S* F, S* Q) {
   const S R2 = xp * xp + u2;
-  const S iR2 = 1.0 / R2;
   const S r = sqrt(R2);
   S ax = 1.0 / r;
+  // 1/R2: a divide on the value path, the square of 1/r on the force path
+  const S iR2 = EXACT ? 1.0 / R2 : ax * ax;
   S a = log(xp + r);
   S b = r;
 #pragma unroll
@@ -267,7 +268,7 @@ __device__ __forceinline__ void mn_grad_t(const smx_potential& p, const S& mGM, 
   const S d2 = (rx * rx + ry * ry) + u * u;
   const S id = 1.0 / sqrt(d2);
   const S f = -mGM * id * id * id;
-  const S grx = f * rx, gry = f * ry, grz = f * u * (rz / zeta);
+  const S grx = f * rx, gry = f * ry, grz = f * u * (rz * (1.0 / zeta));
   gx = (R[0] * grx + R[3] * gry) + R[6] * grz;
   gy = (R[1] * grx + R[4] * gry) + R[7] * grz;
   gz = (R[2] * grx + R[5] * gry) + R[8] * grz;
@@ -290,7 +291,7 @@ __device__ __forceinline__ void t3_grad_t(const smx_potential& p, const S& M, co
   S sz = a * F[1] + dc2 * F[2];
   gx = sx;
   gy = su * 2.0 * y;
-  gz = (sz + su * 2.0 * Z) * (z / ze);
+  gz = (sz + su * 2.0 * Z) * (z * (1.0 / ze));
 }
 
 // grad of V4's bracket ph (dehnen_bar.py:1371-1383): Phi_V4 = -G ph
@@ -323,7 +324,7 @@ __device__ __forceinline__ void v4_grad_t(const smx_potential& p, const S& M, co
   }
   gx = sx;
   gy = su * 2.0 * y;
-  gz = (sz + su * 2.0 * Z) * (z / ze);
+  gz = (sz + su * 2.0 * Z) * (z * ize);
 }
 
 // dehnen_bar.py:1425-1438

@@ -270,12 +270,13 @@ __global__ void copy_out_kernel(int64_t total, const double* __restrict__ x, int
 // Delta products.  R changes slowly from one iteration to the next (its rows
 // move by 1e-4 .. 1e-5 of their size per iteration at the library's
 // conditioning), so after a full product w_b = M R_b an iteration may take
-//   w = w_b + M_hi (R - R_b),
-// reading only the hi plane of M: half the bytes.  Its error,
-// M_lo (R - R_b) ~ 2^-12 |M| |R - R_b|, stays at the full product's 2^-22
-// |M| |R| while every row satisfies max|R_k - R_b,k| <= tau max|R_k| with
-// tau = 2^-10; an iteration where some row does not takes the full product
-// and makes its R the new base.  The choice is made on the device, from the
+//   w = w_b + M_hi (R - R_b)_hi,
+// reading only the hi plane of M (half the bytes) and multiplying fp16 hi
+// parts only (one MFMA instead of three).  Each hi part carries at most
+// 2^-12 relative error, so the delta's error is ~ 2^-11 |M| |R - R_b|: at
+// most twice the full product's 2^-22 |M| |R| while every row satisfies
+// max|R_k - R_b,k| <= tau max|R_k| with tau = 2^-10.  An iteration where some
+// row does not takes the full product and makes its R the new base.  The choice is made on the device, from the
 // row maxima the previous update collected, identically in the product and
 // the update kernel, so the loop needs no host synchronisation.
 //
@@ -381,37 +382,47 @@ struct AdmmStage {
     for (int p = 0; p < PA_; ++p)
       rsc[p] = pow2_scale_for(__uint_as_float(rowmax[k0 + threadIdx.x / TPR + RPP * p]), 14);
   }
+  // A operand chunk (this thread's PA_ rows, 8 FP32 values each) in registers
+  float4 av[PA_][2];
+  __device__ void load_a(const float* __restrict__ R, int64_t Kp, int64_t k0, int64_t i0) {
+    const int t = threadIdx.x;
+    const int64_t base = ((i0 / AG_KP) * Kp + k0) * AG_KP;
+#pragma unroll
+    for (int p = 0; p < PA_; ++p) {
+      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
+      av[p][0] = *reinterpret_cast<const float4*>(R + off);
+      av[p][1] = *reinterpret_cast<const float4*>(R + off + 4);
+    }
+  }
+  // B planes and the A chunk from registers to LDS, A split into fp16 hi / lo
   template <bool FULL, int P>
-  __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P], _Float16 (*Al)[P],
-                        const float* __restrict__ R, int64_t Kp, int64_t k0, int64_t i0) const {
+  __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P],
+                        _Float16 (*Al)[P]) const {
     const int t = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PB_; ++p) {
       *reinterpret_cast<uint4*>(&Bh[t / TPR + RPP * p][8 * (t % TPR)]) = bh[p];
       if (FULL) *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
     }
-    const int64_t base = ((i0 / AG_KP) * Kp + k0) * AG_KP;
 #pragma unroll
     for (int p = 0; p < PA_; ++p) {
-      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
-      const float4 v0 = *reinterpret_cast<const float4*>(R + off);
-      const float4 v1 = *reinterpret_cast<const float4*>(R + off + 4);
-      const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      const float v[8] = {av[p][0].x, av[p][0].y, av[p][0].z, av[p][0].w,
+                          av[p][1].x, av[p][1].y, av[p][1].z, av[p][1].w};
       half8 h, l;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const float sv = v[e] * rsc[p];
         h[e] = static_cast<_Float16>(sv);
-        l[e] = static_cast<_Float16>(sv - static_cast<float>(h[e]));
+        if (FULL) l[e] = static_cast<_Float16>(sv - static_cast<float>(h[e]));
       }
       *reinterpret_cast<half8*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) = h;
-      *reinterpret_cast<half8*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) = l;
+      if (FULL) *reinterpret_cast<half8*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) = l;
     }
   }
 };
 
 // Main loop over the split's chunks.  FULL: hi*hi + hi*lo + lo*hi of R and
-// M; otherwise hi*hi + lo*hi of the operand (D = R - R_b) and M's hi plane.
+// M; otherwise hi*hi of D = R - R_b and M's hi plane.
 template <int KC, bool FULL, int P>
 __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t k0, int64_t j0,
                                                int64_t ch0, int64_t ch1, const float* __restrict__ A,
@@ -424,12 +435,16 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
   if (ch0 < ch1) {
     stg.load_plane(Mh, npad, j0, ch0 * KC, stg.bh);
     if (FULL) stg.load_plane(Ml, npad, j0, ch0 * KC, stg.bl);
-    stg.template store<FULL>(Bh, Bl, Ah, Al, A, Kp, k0, ch0 * KC);
+    stg.load_a(A, Kp, k0, ch0 * KC);
+    stg.template store<FULL>(Bh, Bl, Ah, Al);
   }
   __syncthreads();
   for (int64_t ch = ch0; ch < ch1; ++ch) {
     const bool more = ch + 1 < ch1;
-    if (more) stg.load_plane(Mh, npad, j0, (ch + 1) * KC, stg.bh);
+    if (more) {
+      stg.load_plane(Mh, npad, j0, (ch + 1) * KC, stg.bh);
+      if (!FULL) stg.load_a(A, Kp, k0, (ch + 1) * KC);
+    }
 #pragma unroll
     for (int t = 0; t < KC / 16; ++t) {
       half8 bh[AG_CB], bl[AG_CB];
@@ -442,20 +457,25 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
         const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
-        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
+        half8 al;
+        if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
         for (int cb = 0; cb < AG_CB; ++cb) {
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
-          if (FULL)
+          if (FULL) {
             acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+          }
         }
       }
     }
     __syncthreads();
     if (more) {
-      if (FULL) stg.load_plane(Ml, npad, j0, (ch + 1) * KC, stg.bl);
-      stg.template store<FULL>(Bh, Bl, Ah, Al, A, Kp, k0, (ch + 1) * KC);
+      if (FULL) {
+        stg.load_plane(Ml, npad, j0, (ch + 1) * KC, stg.bl);
+        stg.load_a(A, Kp, k0, (ch + 1) * KC);
+      }
+      stg.template store<FULL>(Bh, Bl, Ah, Al);
     }
     __syncthreads();
   }

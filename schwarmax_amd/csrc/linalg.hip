@@ -778,49 +778,6 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
   }
 }
 
-// ---------------------------------------------------------------------------
-// FP64 maps: out (K, m) = W (K, n) . lib[:, :m]
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void maps_kernel(int K, int64_t n, int64_t m, const double* __restrict__ W,
-                                                   const double* __restrict__ lib, int64_t ld,
-                                                   double* __restrict__ out) {
-  constexpr int TM = 16, TN = 64, TK = 32;
-  __shared__ double Ws[TK][TM + 1];
-  __shared__ double Ls[TK][TN + 1];
-  const int k0 = blockIdx.y * TM;
-  const int64_t j0 = blockIdx.x * (int64_t)TN;
-  const int tid = threadIdx.x;
-  const int tj = tid & 63, tk = tid >> 6;   // 4 row groups of 4
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t i0 = 0; i0 < n; i0 += TK) {
-    for (int e = tid; e < TK * TM; e += 256) {
-      const int ii = e % TK, kk = e / TK;
-      const int64_t gi = i0 + ii;
-      Ws[ii][kk] = (gi < n && k0 + kk < K) ? W[(int64_t)(k0 + kk) * n + gi] : 0.0;
-    }
-    for (int e = tid; e < TK * TN; e += 256) {
-      const int jj = e % TN, ii = e / TN;
-      const int64_t gi = i0 + ii, gj = j0 + jj;
-      Ls[ii][jj] = (gi < n && gj < m) ? lib[gi * ld + gj] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int ii = 0; ii < TK; ++ii) {
-      const double l = Ls[ii][tj];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] += Ws[ii][tk * 4 + r] * l;
-    }
-    __syncthreads();
-  }
-  const int64_t gj = j0 + tj;
-  if (gj >= m) return;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int k = k0 + tk * 4 + r;
-    if (k < K) out[(int64_t)k * m + gj] = acc[r];
-  }
-}
-
 static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct NnlsWs {
@@ -1422,12 +1379,13 @@ int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream) {
 int smx_model_maps(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const double* W,
                    int64_t K, double* maps, void* stream) {
   SMX_REQUIRE(lib && W && maps && n > 0 && m > 0 && K > 0 && ld_lib >= m, "smx_model_maps: bad arguments");
-  dim3 grid((unsigned)((m + 63) / 64), (unsigned)((K + 15) / 16));
-  stage_begin(ST_MAPS, as_stream(stream));
-  hipLaunchKernelGGL(maps_kernel, grid, dim3(256), 0, as_stream(stream), (int)K, n, m, W, lib, ld_lib,
-                     maps);
-  SMX_LAUNCHED();
-  stage_end(ST_MAPS, as_stream(stream), 1);
+  // row-major out (K, m) = W (K, n) lib[:, :m] is the column-major product
+  // out^T (m, K) = lib[:, :m]^T (m, n, leading dimension ld_lib) W^T (n, K):
+  // one FP64 GEMM on the matrix cores
+  hipStream_t st = as_stream(stream);
+  stage_begin(ST_MAPS, st);
+  if (int e = blas_dgemm(st, false, false, m, K, n, 1.0, lib, ld_lib, W, n, 0.0, maps, m)) return e;
+  stage_end(ST_MAPS, st, 1);
   return 0;
 }
 

@@ -104,15 +104,10 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
       for (int j = 0; j < 6; ++j) y[j] = M.wb[6 * q + j];
       deriv(p, s.Omega, y, k1);
       const double EJ0 = jacobi(p, s.Omega, y);
-      double t = 0.0, dt = M.dt0[q], T_acc = 0.0;
-      const int nsym = g.symmetry ? 4 : 1;
-      for (int step = 0; step < s.N_max; ++step) {
-        const double used = bs3_attempt(p, s, t, y, k1, dt);
-        if (used != 0.0) {
-          bin_sample(g, bm, v0, sd, y, used, nsym, rz, ap, P);
-          T_acc += nsym * used;
-        }
-      }
+      double t = 0.0, dt = M.dt0[q];
+      const double T_acc =
+          g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1)
+                     : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1);
       const double EJ = jacobi(p, s.Omega, y);
       valid = ((fabs(EJ / EJ0 - 1.0) < 0.1) && (t > M.T_total[q] / 10.0)) ? 1.0 : 0.0;
       T = T_acc + kEps;

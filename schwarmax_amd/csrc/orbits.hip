@@ -107,22 +107,13 @@ __global__ __launch_bounds__(64, SMX_INTEGRATOR_WAVES) void integrate_library_ke
   for (int j = 0; j < 6; ++j) y[j] = wb[6 * i + j];
   deriv(p, s.Omega, y, k1);
   const double EJ0 = jacobi(p, s.Omega, y);
-  double t = 0.0, dt = dt0[i], T_acc = 0.0;
-  const int nsym = g.symmetry ? 4 : 1;
-  for (int step = 0; step < s.N_max; ++step) {
-    if (ckpt != nullptr && step % ckpt_every == 0) {
-      double* c = ckpt + ((int64_t)(step / ckpt_every) * n_int + i) * 8;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) c[j] = y[j];
-      c[6] = dt;
-      c[7] = t;
-    }
-    const double used = bs3_attempt(p, s, t, y, k1, dt);
-    if (used != 0.0) {
-      bin_sample(g, bm, v0, sd, y, used, nsym, rz, ap, P);
-      T_acc += nsym * used;
-    }
-  }
+  double t = 0.0, dt = dt0[i];
+  double* ck = ckpt != nullptr ? ckpt + i * 8 : nullptr;
+  const double T_acc =
+      g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, ck, n_int * 8,
+                                        ckpt_every)
+                 : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, ck, n_int * 8,
+                                        ckpt_every);
   const double EJ = jacobi(p, s.Omega, y);
   const bool ok = (fabs(EJ / EJ0 - 1.0) < 0.1) && (t > T_total[i] / 10.0);
   valid_out[i] = ok ? 1.0 : 0.0;

@@ -211,6 +211,12 @@ def bench_batched(args):
                    'admm_iters': T},
         'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
     }
+    from schwarmax_amd import tune
+    out['autotune'] = [{'shape': list(r['shape']), 'occupancy': r['occupancy'],
+                        'default': r['default'],
+                        'ms_full_delta_weighted': {str(o): [round(x, 4) for x in v]
+                                                   for o, v in r['ms'].items()}}
+                       for r in tune._RESULTS.values()]
     if args.compare:
         t0 = time.time()
         for m in range(M):

@@ -247,6 +247,20 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
 int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* workspace,
                                 int32_t* full_products, void* stream);
 
+/* ---- ADMM product tuning (no reference counterpart: the knob the
+ * autotuner in schwarmax_amd/tune.py sets) ------------------------------- */
+/* Workgroups per CU that the ADMM product's split-K rule targets (1 ..
+ * 4; 0 restores the default, 2).  Process-wide, atomic; applies to solves
+ * started afterwards (workspaces are sized for the largest setting). */
+int smx_admm_set_occupancy(int32_t workgroups_per_cu);
+int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
+/* Mean time of `reps` ADMM product launches (full = 1: the full split
+ * product; 0: the delta product) on the buffers of a workspace that a
+ * smx_nnls_admm call of the same (n, m, K) has filled, with the current
+ * occupancy.  Synchronises the stream. */
+int smx_admm_product_time(int64_t n, int64_t m, int64_t K, void* workspace, int32_t full,
+                          int32_t reps, float* ms_per_launch, void* stream);
+
 /* ---- dense building blocks (no reference counterpart: the reference calls
  * XLA's dot / cholesky / cho_solve; exposed for tests) ------------------- */
 /* C = alpha A B^T + beta C on MFMA, A (M,K) and B (N,K) row-major (k

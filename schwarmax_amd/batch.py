@@ -42,11 +42,13 @@ def _groups(n_orbits, row_bytes, budget):
 
 def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
                            sigma_amplify=1.0, schedule=None, nnls_maxiter=200, nnls_tol=0.0,
-                           library_budget_bytes=48 << 30, return_aux=False):
+                           library_budget_bytes=48 << 30, autotune=True, return_aux=False):
     """logL for each row of thetas (M, 12); returns a CUDA tensor (M,).
 
     n_orbits: per-model orbit counts (default: all orbits for every model).
     schedule: one schedule dict, or a list of M (their N_max may differ).
+    autotune: pick the ADMM product's occupancy for the batch's median shape
+    first (``tune.ensure_tuned``, once per process and shape bucket).
     """
     thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
     M = thetas.shape[0]
@@ -56,6 +58,11 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
     n_orbits = [n_all] * M if n_orbits is None else [int(v) for v in n_orbits]
     if len(n_orbits) != M or min(n_orbits) < 1 or max(n_orbits) > n_all:
         raise ValueError("n_orbits must hold M counts in [1, number of orbits in dict_data]")
+    if autotune:
+        from . import tune
+        m_rows = int(Rzphi_n_tot) + 5 * int(num_Vbin)
+        K = np.asarray(dict_data['XY_standard_normal']).shape[0]
+        tune.ensure_tuned(int(np.median(n_orbits)), m_rows, K)
     scheds = schedule if isinstance(schedule, (list, tuple)) else [schedule] * M
     scheds = [MD.schedule_of(s) for s in scheds]
     r = int(scheds[0]['n_realizations'])

@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include <mutex>
+#include <atomic>
 #include <rocblas/rocblas.h>
 
 #include "common.hpp"
@@ -295,13 +296,14 @@ constexpr int AG_CB = AG_BN / 128;   // 32-column blocks per wave
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
 constexpr float AG_TAU = 1.0f / 1024.0f;   // delta-product bound max|R - R_b| / max|R| per row
 
-// Full product this iteration?  Forced, or some row k < K has max|D_k| >
+// Full product this iteration?  force > 0: yes; force < 0: no (timing of the
+// delta product only); otherwise yes when some row k < K has max|D_k| >
 // tau max|R_k| (D = R - R_b).  Block-uniform; every thread must call it.
 __device__ __forceinline__ bool admm_full_product(int force, int64_t K,
                                                   const unsigned* __restrict__ rmax,
                                                   const unsigned* __restrict__ dmax) {
-  int need = force;
-  if (!need)
+  int need = force > 0 ? 1 : 0;
+  if (force == 0)
     for (int64_t k = threadIdx.x; k < K; k += blockDim.x)
       if (__uint_as_float(dmax[k]) > AG_TAU * __uint_as_float(rmax[k])) need = 1;
   return __syncthreads_or(need) != 0;
@@ -807,16 +809,26 @@ struct NnlsWs {
   int64_t bytes;
 };
 
-// splits of the reduction so that the ADMM product fills the chip once
-// (AG_OCC workgroups on each of the 256 CUs)
-static int admm_splits(int64_t npad, int64_t Kp) {
+// Splits of the ADMM product's reduction: enough workgroups for `occ` on each
+// of the 256 CUs.  occ is the product's tuning knob (smx_admm_set_occupancy,
+// set by the autotuner; default AG_OCC, the workgroups per CU that the
+// kernel's LDS and registers allow at once).  Workspaces size the partial
+// buffer for occ = AG_OCC_MAX, so a later setting never overruns one.
+constexpr int AG_OCC_MAX = 4;
+static std::atomic<int> g_admm_occ{AG_OCC};
+
+static int admm_splits_for(int64_t npad, int64_t Kp, int occ) {
   const int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);
-  int64_t S = (256 * AG_OCC) / (tiles > 0 ? tiles : 1);
+  int64_t S = (256 * (int64_t)occ) / (tiles > 0 ? tiles : 1);
   if (S < 1) S = 1;
   if (S > 16) S = 16;
   const int64_t nch = npad / AG_KP;
   if (S > nch) S = nch;
   return static_cast<int>(S);
+}
+
+static int admm_splits(int64_t npad, int64_t Kp) {
+  return admm_splits_for(npad, Kp, g_admm_occ.load(std::memory_order_relaxed));
 }
 
 static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
@@ -847,7 +859,8 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Mh = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Ml = reinterpret_cast<_Float16*>(take(w.npad * w.npad * 2));
   w.Rp = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
-  w.part = reinterpret_cast<float*>(take((int64_t)w.S * w.Kp * w.npad * 4));
+  w.part = reinterpret_cast<float*>(
+      take((int64_t)admm_splits_for(w.npad, w.Kp, AG_OCC_MAX) * w.Kp * w.npad * 4));
   w.rmax = reinterpret_cast<unsigned*>(take(6 * w.Kp * 4));
   w.Rb = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
@@ -1307,6 +1320,47 @@ int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* wor
   SMX_HIP(hipMemcpyAsync(&v, w.full_count, 4, hipMemcpyDeviceToHost, st));
   SMX_HIP(hipStreamSynchronize(st));
   *full_products = static_cast<int32_t>(v);
+  return 0;
+}
+
+int smx_admm_set_occupancy(int32_t workgroups_per_cu) {
+  SMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= AG_OCC_MAX,
+              "smx_admm_set_occupancy: workgroups_per_cu must be 0 (default) .. %d", AG_OCC_MAX);
+  g_admm_occ.store(workgroups_per_cu == 0 ? AG_OCC : workgroups_per_cu, std::memory_order_relaxed);
+  return 0;
+}
+
+int smx_admm_get_occupancy(int32_t* workgroups_per_cu) {
+  SMX_REQUIRE(workgroups_per_cu, "smx_admm_get_occupancy: null pointer");
+  *workgroups_per_cu = g_admm_occ.load(std::memory_order_relaxed);
+  return 0;
+}
+
+int smx_admm_product_time(int64_t n, int64_t m, int64_t K, void* workspace, int32_t full,
+                          int32_t reps, float* ms_per_launch, void* stream) {
+  SMX_REQUIRE(workspace && ms_per_launch && n > 0 && m > 0 && K > 0 && reps > 0,
+              "smx_admm_product_time: bad arguments");
+  hipStream_t st = as_stream(stream);
+  NnlsWs w = carve(workspace, n, m, K);
+  const int64_t npad = w.npad, Kp = w.Kp;
+  const int64_t nch = npad / AG_KP;
+  const int cps = static_cast<int>((nch + w.S - 1) / w.S);
+  dim3 ggrid((unsigned)((npad / AG_BN) * w.S * (Kp / AG_KR)));
+  hipEvent_t e0, e1;
+  SMX_HIP(hipEventCreate(&e0));
+  SMX_HIP(hipEventCreate(&e1));
+  SMX_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
+                       w.rmax, w.D, w.rmax + 3 * Kp, full ? 1 : -1, w.Mh, w.Ml, w.part);
+  SMX_LAUNCHED();
+  SMX_HIP(hipEventRecord(e1, st));
+  SMX_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  SMX_HIP(hipEventDestroy(e0));
+  SMX_HIP(hipEventDestroy(e1));
+  *ms_per_launch = ms / static_cast<float>(reps);
   return 0;
 }
 

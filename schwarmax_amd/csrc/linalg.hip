@@ -220,16 +220,6 @@ static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, con
 // ---------------------------------------------------------------------------
 // design matrix, Gram regularisation, trace
 // ---------------------------------------------------------------------------
-// Ut[i, j] = lib[i, j] * scale[j] for j < m, 0 for m <= j < ldu
-__global__ void build_ut_kernel(const double* __restrict__ lib, int64_t ld_lib, int64_t n,
-                                int64_t m, int64_t ldu, const double* __restrict__ scale,
-                                float* __restrict__ Ut) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= n * ldu) return;
-  const int64_t i = idx / ldu, j = idx - i * ldu;
-  Ut[idx] = (j < m) ? static_cast<float>(lib[i * ld_lib + j] * scale[j]) : 0.f;
-}
-
 __global__ void to_float_kernel(const double* __restrict__ src, int64_t rows, int64_t cols,
                                 int64_t ld_dst, float* __restrict__ dst) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -784,7 +774,10 @@ static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct NnlsWs {
   float* Ut;      // n x ldu
-  float* Q;       // n x n FP32 Gram
+  _Float16* Uh;   // gpad x mpad, fp16 hi plane of Ut (row i scaled by us[i]), Gram-blocked
+  _Float16* Ul;   // gpad x mpad, fp16 lo plane
+  float* us;      // gpad power-of-two row scales
+  int64_t gpad, mpad;
   double* Q64;    // n x n FP64, inverted in place
   double* work64; // n*n + n work of the recursive inverse
   float* Yf;      // K x ldu
@@ -842,7 +835,11 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
     return q;
   };
   w.Ut = reinterpret_cast<float*>(take(n * ldu * 4));
-  w.Q = reinterpret_cast<float*>(take(n * n * 4));
+  w.gpad = align_up(n, 256);
+  w.mpad = align_up(m, 32);
+  w.Uh = reinterpret_cast<_Float16*>(take(w.gpad * w.mpad * 2));
+  w.Ul = reinterpret_cast<_Float16*>(take(w.gpad * w.mpad * 2));
+  w.us = reinterpret_cast<float*>(take(w.gpad * 4));
   w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
   w.work64 = reinterpret_cast<double*>(take((n * n + n) * 8));
   w.Yf = reinterpret_cast<float*>(take(K * ldu * 4));
@@ -873,6 +870,155 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
 __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Gram Q = U^T U on fp16 MFMA with hi/lo splits (three products, FP32
+// accumulation), written straight into the FP64 matrix that is inverted.
+// Row i of Ut (orbit i, m data rows) is scaled by a power of two s_i taken
+// from its maximum, so hi + lo holds it to 2^-22 relative and both stay in
+// fp16's normal range; Q_ij = acc_ij / (s_i s_j).  At plain-20k this Gram
+// moves logL by 3.8e-7 against an FP64 Gram (an FP32 Gram: 2.8e-7), far
+// below the ADMM product's share of the error.
+// Planes are blocked like the ADMM product's: tile (128-row block I, 32-index
+// chunk c) = rows 128 I + r, indices 32 c + q at ((I nchk + c) 128 + r) 32 + q.
+// ---------------------------------------------------------------------------
+constexpr int GR_TM = 128;   // rows (A) per tile; the B operand is two such blocks
+
+__device__ __forceinline__ int64_t g_tile_index(int64_t i, int64_t k, int64_t nchk) {
+  return ((i / GR_TM * nchk + k / AG_KP) * GR_TM + i % GR_TM) * AG_KP + k % AG_KP;
+}
+
+// One workgroup per (padded) row i: Ut[i, :] = lib[i, :m] * scale (FP32, for
+// the right-hand sides) and its scaled fp16 hi / lo planes.
+__global__ __launch_bounds__(256) void build_ut_split_kernel(
+    const double* __restrict__ lib, int64_t ld_lib, int64_t n, int64_t m, int64_t ldu,
+    int64_t mpad, const double* __restrict__ scale, float* __restrict__ Ut,
+    _Float16* __restrict__ Uh, _Float16* __restrict__ Ul, float* __restrict__ us) {
+  __shared__ float red[4];
+  const int64_t i = blockIdx.x;
+  const int64_t nchk = mpad / AG_KP;
+  float mx = 0.f;
+  if (i < n)
+    for (int64_t j = threadIdx.x; j < m; j += blockDim.x)
+      mx = fmaxf(mx, static_cast<float>(fabs(lib[i * ld_lib + j] * scale[j])));
+  mx = block_max_256(mx, red);
+  const float sc = pow2_scale_for(mx, 14);
+  if (threadIdx.x == 0) us[i] = sc;
+  for (int64_t j = threadIdx.x; j < mpad; j += blockDim.x) {
+    const double u = (i < n && j < m) ? lib[i * ld_lib + j] * scale[j] : 0.0;
+    if (i < n && j < ldu) Ut[i * ldu + j] = static_cast<float>(u);
+    const double v = u * static_cast<double>(sc);
+    const _Float16 h = static_cast<_Float16>(v);
+    const int64_t d = g_tile_index(i, j, nchk);
+    Uh[d] = h;
+    Ul[d] = static_cast<_Float16>(v - static_cast<double>(h));
+  }
+}
+
+// Lower-triangle tiles: row block I (128 rows), column block J (256 columns)
+// with 256 J <= 128 I, enumerated row block by row block.  Row block 2p holds
+// p + 1 tiles starting at p (p + 1), row block 2p + 1 p + 1 tiles starting at
+// (p + 1)^2.
+__device__ __forceinline__ void gram_tile(int64_t t, int64_t& I, int64_t& J) {
+  int64_t p = static_cast<int64_t>((sqrt(4.0 * static_cast<double>(t) + 1.0) - 1.0) * 0.5);
+  while (p > 0 && p * (p + 1) > t) --p;
+  while ((p + 1) * (p + 2) <= t) ++p;
+  if (t < (p + 1) * (p + 1)) {
+    I = 2 * p;
+    J = t - p * (p + 1);
+  } else {
+    I = 2 * p + 1;
+    J = t - (p + 1) * (p + 1);
+  }
+}
+
+static int64_t gram_tiles(int64_t gpad) {
+  const int64_t nrb = gpad / GR_TM;     // even: gpad is a multiple of 256
+  const int64_t q = nrb / 2;
+  return q * (q + 1);
+}
+
+// One (128 x 256) tile of Q per workgroup (4 waves of 128 x 64), 32-index
+// chunks staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep step; writes
+// Q[i][j] for j <= i < n.
+__global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
+    int64_t n, int64_t mpad, const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
+    const float* __restrict__ us, double* __restrict__ Q) {
+  constexpr int P = AG_KP + 8;
+  __shared__ __align__(16) _Float16 Bh[2 * GR_TM][P];
+  __shared__ __align__(16) _Float16 Bl[2 * GR_TM][P];
+  __shared__ __align__(16) _Float16 Ah[GR_TM][P];
+  __shared__ __align__(16) _Float16 Al[GR_TM][P];
+  int64_t I, J;
+  gram_tile(blockIdx.x, I, J);
+  const int64_t i0 = I * GR_TM, j0 = J * 2 * GR_TM;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 5, c = lane & 31;
+  const int64_t nchk = mpad / AG_KP;
+  const int64_t tileA = I * nchk, tileB0 = 2 * J * nchk, tileB1 = (2 * J + 1) * nchk;
+  const int r0 = tid >> 2, seg = 8 * (tid & 3);   // rows r0, r0 + 64 of a 128-row tile
+  f32x16 acc[4][AG_CB];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < AG_CB; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  for (int64_t ch = 0; ch < nchk; ++ch) {
+    // stage chunk ch: each plane tile is 512 16-byte runs, thread t moves t and t + 256
+    const int64_t oa = ((tileA + ch) * GR_TM) * AG_KP;
+    const int64_t ob0 = ((tileB0 + ch) * GR_TM) * AG_KP;
+    const int64_t ob1 = ((tileB1 + ch) * GR_TM) * AG_KP;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t e = (int64_t)(r0 + 64 * h) * AG_KP + seg;
+      *reinterpret_cast<uint4*>(&Ah[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Uh + oa + e);
+      *reinterpret_cast<uint4*>(&Al[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Ul + oa + e);
+      *reinterpret_cast<uint4*>(&Bh[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Uh + ob0 + e);
+      *reinterpret_cast<uint4*>(&Bl[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Ul + ob0 + e);
+      *reinterpret_cast<uint4*>(&Bh[GR_TM + r0 + 64 * h][seg]) =
+          *reinterpret_cast<const uint4*>(Uh + ob1 + e);
+      *reinterpret_cast<uint4*>(&Bl[GR_TM + r0 + 64 * h][seg]) =
+          *reinterpret_cast<const uint4*>(Ul + ob1 + e);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < AG_KP / 16; ++t) {
+      half8 bh[AG_CB], bl[AG_CB];
+#pragma unroll
+      for (int cb = 0; cb < AG_CB; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
+        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
+      }
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
+        const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
+#pragma unroll
+        for (int cb = 0; cb < AG_CB; ++cb) {
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < AG_CB; ++cb) {
+      const int64_t j = j0 + wave * 64 + cb * 32 + c;
+      const double sj = (j < n) ? static_cast<double>(us[j]) : 1.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = i0 + rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        if (i < n && j <= i)
+          Q[i * n + j] = static_cast<double>(acc[rb][cb][r]) / (static_cast<double>(us[i]) * sj);
+      }
+    }
 }
 
 
@@ -1082,11 +1228,7 @@ static int spd_inverse64(hipStream_t st, double* A, int64_t n, int64_t lda, doub
   return 0;
 }
 
-// Q (FP32, full) -> FP64 with Q += reg I, rho = tr(Q)/n, Q += rho I (nnls.py:115-121)
-__global__ void widen_kernel(const float* __restrict__ Q, int64_t nn, double* __restrict__ D) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx < nn) D[idx] = static_cast<double>(Q[idx]);
-}
+// Q64 (lower triangle): Q += reg I, rho = tr(Q)/n, Q += rho I (nnls.py:115-121)
 
 __global__ __launch_bounds__(1024) void regularise64_kernel(double* __restrict__ Q, int64_t n,
                                                             double reg, double* __restrict__ rho_out,
@@ -1114,6 +1256,24 @@ __global__ __launch_bounds__(1024) void regularise64_kernel(double* __restrict__
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Q[i * n + i] = Q[i * n + i] + rho;
 }
 
+// Q64 (lower triangle) = U^T U + (lambda/n) I + rho I with rho = tr(U^T U +
+// (lambda/n) I)/n (nnls.py:115-121), U[j, i] = lib[i, j] row_scale[j]; also
+// the FP32 Ut for the right-hand sides.
+static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64_t ld_lib,
+                       int64_t n, int64_t m, const double* row_scale, double lambda_reg) {
+  const int64_t ldu = align_up(m, 16);
+  hipLaunchKernelGGL(build_ut_split_kernel, dim3((unsigned)w.gpad), dim3(256), 0, st, lib, ld_lib,
+                     n, m, ldu, w.mpad, row_scale, w.Ut, w.Uh, w.Ul, w.us);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_tiles(w.gpad)), dim3(256), 0, st, n,
+                     w.mpad, w.Uh, w.Ul, w.us, w.Q64);
+  SMX_LAUNCHED();
+  hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
+                     lambda_reg / static_cast<double>(n), w.rho64, w.rho);
+  SMX_LAUNCHED();
+  return 0;
+}
+
 }  // namespace smx
 
 using namespace smx;
@@ -1135,19 +1295,10 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   NnlsWs w = carve(workspace, n, m, K);
   const int64_t ldu = align_up(m, 16);
   stage_begin(ST_GRAM, st);
-  hipLaunchKernelGGL(build_ut_kernel, dim3(blocks_for(n * ldu, 256)), dim3(256), 0, st, lib, ld_lib,
-                     n, m, ldu, row_scale, w.Ut);
-  SMX_LAUNCHED();
-  // Q = U^T U: lower half on MFMA (FP32), mirrored, widened to FP64
-  if (int e = gemm_nt<float>(st, n, n, ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1)) return e;
-  mirror_lower<float>(st, w.Q, n, n);
-  SMX_LAUNCHED();
-  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
-  SMX_LAUNCHED();
-  hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
-                     lambda_reg / static_cast<double>(n), w.rho64, w.rho);
-  SMX_LAUNCHED();
-  stage_end(ST_GRAM, st, 5);
+  // Q = U^T U (lower triangle, FP64 out of fp16-split MFMA), + lambda/n I,
+  // rho = tr/n, + rho I
+  if (int e = form_gram64(st, w, lib, ld_lib, n, m, row_scale, lambda_reg)) return e;
+  stage_end(ST_GRAM, st, 3);
   // Q + rho I <- (Q + rho I)^-1 in place, FP64
   stage_begin(ST_FACTOR, st);
   if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
@@ -1286,18 +1437,8 @@ static int nnls_build_operator(hipStream_t st, const NnlsWs& w, const double* li
                                int64_t n, int64_t m, const double* row_scale, double lambda_reg) {
   const int64_t ldu = align_up(m, 16);
   stage_begin(ST_GRAM, st);
-  hipLaunchKernelGGL(build_ut_kernel, dim3(blocks_for(n * ldu, 256)), dim3(256), 0, st, lib, ld_lib,
-                     n, m, ldu, row_scale, w.Ut);
-  SMX_LAUNCHED();
-  if (int e = gemm_nt<float>(st, n, n, ldu, 1.f, w.Ut, ldu, w.Ut, ldu, 0.f, w.Q, n, 1)) return e;
-  mirror_lower<float>(st, w.Q, n, n);
-  SMX_LAUNCHED();
-  hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n * n, 256)), dim3(256), 0, st, w.Q, n * n, w.Q64);
-  SMX_LAUNCHED();
-  hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
-                     lambda_reg / static_cast<double>(n), w.rho64, w.rho);
-  SMX_LAUNCHED();
-  stage_end(ST_GRAM, st, 5);
+  if (int e = form_gram64(st, w, lib, ld_lib, n, m, row_scale, lambda_reg)) return e;
+  stage_end(ST_GRAM, st, 3);
   stage_begin(ST_FACTOR, st);
   if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
   hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(w.npad * w.npad, 256)), dim3(256), 0, st,

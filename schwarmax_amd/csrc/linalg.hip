@@ -668,7 +668,7 @@ struct AdmmIter {
   float* R;                 // Kp x npad blocked: this iteration's R, replaced by the next
   float* Rb;                // Kp x npad blocked: R at the last full product
   float* D;                 // Kp x npad blocked: next R - R_b
-  double* wb;               // K x n: w of the last full product
+  float* wb;                // K x n: w of the last full product
   const unsigned* rcur;     // max |R_k| of this iteration's R
   unsigned* rnext;
   unsigned* rclear;
@@ -702,7 +702,9 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
   if (i0 < n) {
     const int64_t ri = r_tile_index(k, i0, Kp);
     float rcur[V], rb[V], rn[V], dn[V];
-    load_vec<V>(b.R + ri, rcur);
+    // this iteration's R: the new base after a full product, and kept by an
+    // inactive row; an active row after a delta product does not need it
+    if (full || !active[k]) load_vec<V>(b.R + ri, rcur);
     if (full) {
       store_vec<V>(b.Rb + ri, rcur);
 #pragma unroll
@@ -724,14 +726,21 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
       double w[V], xo[V], xn[V];
       float cv[V];
       load_vec<V>(c + idx, cv);
+      // w_b in FP32 holds the full product's w exactly: an FP32 sum times a
+      // power of two
+      float wf[V];
       if (full) {
 #pragma unroll
-        for (int e = 0; e < V; ++e) w[e] = static_cast<double>(sum[e]) * inv;
-        store_vec<V>(b.wb + idx, w);
+        for (int e = 0; e < V; ++e) {
+          wf[e] = sum[e] * static_cast<float>(inv);
+          w[e] = static_cast<double>(wf[e]);
+        }
+        store_vec<V>(b.wb + idx, wf);
       } else {
-        load_vec<V>(b.wb + idx, w);
+        load_vec<V>(b.wb + idx, wf);
 #pragma unroll
-        for (int e = 0; e < V; ++e) w[e] += static_cast<double>(sum[e]) * inv;
+        for (int e = 0; e < V; ++e)
+          w[e] = static_cast<double>(wf[e]) + static_cast<double>(sum[e]) * inv;
       }
       if (first) {
 #pragma unroll
@@ -793,7 +802,7 @@ struct NnlsWs {
   float* Rp;        // Kp x npad, blocked (r_tile_index)
   float* Rb;        // Kp x npad, blocked: R at the last full product
   float* D;         // Kp x npad, blocked: R - Rb
-  double* wb;       // K x n: w of the last full product
+  float* wb;        // K x n: w of the last full product
   unsigned* full_count;  // full ADMM products taken by the last solve
   float* part;      // S x Kp x npad
   unsigned* rmax;   // 6 x Kp: three rotating row-maximum sets of |R|, then three of |D|
@@ -861,7 +870,7 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.rmax = reinterpret_cast<unsigned*>(take(6 * w.Kp * 4));
   w.Rb = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
-  w.wb = reinterpret_cast<double*>(take(K * n * 8));
+  w.wb = reinterpret_cast<float*>(take(K * n * 4));
   w.full_count = reinterpret_cast<unsigned*>(take(4));
   w.bytes = off;
   return w;

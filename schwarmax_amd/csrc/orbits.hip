@@ -62,7 +62,10 @@ __global__ void density_eval_kernel(smx_potential p, const double* __restrict__ 
 // Jeans initial conditions (jeans.py:9-94) + jitter / time scales
 // (model.py:117-169).  One 64-lane wavefront per star.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void initial_conditions_kernel(
+#ifndef SMX_IC_WAVES
+#define SMX_IC_WAVES 2   // waves per SIMD the IC kernel's registers are sized for (2: 10.0 ms, 1: 11.0 ms at plain-20k)
+#endif
+__global__ __launch_bounds__(256, SMX_IC_WAVES) void initial_conditions_kernel(
     smx_potential p, const double* __restrict__ w0, const double* __restrict__ normals,
     const double* __restrict__ uniforms, int64_t n, int r, int r_total, double N_step,
     double N_dyn, double* __restrict__ wb, double* __restrict__ T_total, double* __restrict__ dt0) {

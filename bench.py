@@ -151,7 +151,8 @@ def bench_jackknife(args):
     dt = time.time() - t0
     stages = _lib.timing_read()
     _lib.timing_enable(False)
-    log(f'delta chi2 {float(v):.6g}; {dt / args.steps:.2f} s; Neumann terms {aux["neumann_terms"]}')
+    log(f'delta chi2 {float(v):.6g}; {dt / args.steps:.2f} s; Neumann terms {aux["neumann_terms"]}; '
+        f'ADMM operator products {aux["products"]} ({aux["full_products"]} full)')
     out = {
         'metric': 'jackknife delta-chi2 evals/s (one group per aperture, 1000 replicates)',
         'value': args.steps / dt, 'unit': 'evals/s', 'n_gpus': 1, 'steps': args.steps,
@@ -160,6 +161,7 @@ def bench_jackknife(args):
         'config': {'workload': args.config, 'orbits': int(dd['w0'].shape[0]), 'replicates': B,
                    'apertures': B, 'rzphi_cells': P, 'admm_iters': 200,
                    'neumann_terms': aux['neumann_terms'],
+                   'operator_products': aux['products'], 'full_products': aux['full_products'],
                    'max_shift_over_rho': float(aux['delta'].max()) / aux['rho']},
         'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
     }

@@ -199,6 +199,12 @@ int smx_nnls_operator(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
  * product: fp16 hi/lo splits, three MFMAs, FP32 accumulation). */
 int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
                             int64_t K, float* out, void* stream);
+/* out (K, n) = D_hi M_hi: the fp16 hi parts only, one MFMA (the ADMM's delta
+ * product).  For D = R - R_b, a change of an input R_b whose full product
+ * the caller holds, out + (R_b M) equals R M to twice the full product's
+ * error bound while max|D_k| <= 2^-10 max|R_k| on every row. */
+int smx_nnls_operator_apply_hi(int64_t n, int64_t m, int64_t K_max, void* workspace,
+                               const float* D, int64_t K, float* out, void* stream);
 
 /* ---- batched evaluation (many theta, one launch per stage) -------------- */
 /* Workspace for smx_build_library_batch: the persistent grid's slab pool plus

@@ -1533,8 +1533,10 @@ int smx_nnls_operator(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
   return 0;
 }
 
-int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
-                            int64_t K, float* out, void* stream) {
+// out = R M (hi_only = 0: the full split product) or R_hi M_hi (hi_only = 1:
+// the delta product's arithmetic, for R a small change of an earlier input).
+static int operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
+                          int64_t K, float* out, int hi_only, void* stream) {
   SMX_REQUIRE(workspace && R && out, "smx_nnls_operator_apply: null pointer");
   SMX_REQUIRE(n > 0 && m > 0 && K > 0 && K <= K_max, "smx_nnls_operator_apply: bad sizes");
   hipStream_t st = as_stream(stream);
@@ -1549,14 +1551,25 @@ int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace
   SMX_LAUNCHED();
   dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM_PRODUCT, st);
+  // the padded rows serve as R (full) or as D (delta), with the same row maxima
   hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
-                     w.rmax, (const float*)nullptr, (const unsigned*)nullptr, 1, w.Mh, w.Ml, w.part);
+                     w.rmax, w.Rp, w.rmax, hi_only ? -1 : 1, w.Mh, w.Ml, w.part);
   SMX_LAUNCHED();
   stage_end(ST_ADMM_PRODUCT, st, 1);
   hipLaunchKernelGGL(apply_out_kernel, dim3(blocks_for(K * n, 256)), dim3(256), 0, st, w.part, S,
                      Kp, npad, K, n, w.rho, w.rmax, out);
   SMX_LAUNCHED();
   return 0;
+}
+
+int smx_nnls_operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, const float* R,
+                            int64_t K, float* out, void* stream) {
+  return operator_apply(n, m, K_max, workspace, R, K, out, 0, stream);
+}
+
+int smx_nnls_operator_apply_hi(int64_t n, int64_t m, int64_t K_max, void* workspace,
+                               const float* D, int64_t K, float* out, void* stream) {
+  return operator_apply(n, m, K_max, workspace, D, K, out, 1, stream);
 }
 
 int smx_gemm_nt_f32(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,

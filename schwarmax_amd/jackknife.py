@@ -50,17 +50,47 @@ class _Operator:
                                        _lib.stream_ptr()), 'smx_nnls_operator')
         self.rho = float(rho.item())
 
-    def apply(self, R):
-        """R (K, n) -> R M (FP32 in, FP64 out), K <= K_max per call."""
+    def apply(self, R, hi_only=False):
+        """R (K, n) -> R M (FP32 in, FP64 out), K <= K_max per call; hi_only:
+        R_hi M_hi (the delta product)."""
         L = _lib.lib()
+        fn = L.smx_nnls_operator_apply_hi if hi_only else L.smx_nnls_operator_apply
         R = R.to(torch.float32).contiguous()
         out = torch.empty_like(R)
         for k0 in range(0, R.shape[0], self.K_max):
             k1 = min(k0 + self.K_max, R.shape[0])
-            _lib.check(L.smx_nnls_operator_apply(self.n, self.m, self.K_max, _lib.ptr(self.ws),
-                                                 _lib.ptr(R[k0:k1]), k1 - k0, _lib.ptr(out[k0:k1]),
-                                                 _lib.stream_ptr()), 'smx_nnls_operator_apply')
+            _lib.check(fn(self.n, self.m, self.K_max, _lib.ptr(self.ws), _lib.ptr(R[k0:k1]), k1 - k0,
+                          _lib.ptr(out[k0:k1]), _lib.stream_ptr()), 'smx_nnls_operator_apply')
         return out.double()
+
+
+class _DeltaApply:
+    """Products R_t M of a slowly changing input sequence R_t (one per ADMM
+    iteration): a full product when the input has moved by more than TAU of
+    some row's size since the last full one, otherwise that product plus the
+    delta product (R_t - R_b)_hi M_hi, as the solver's own ADMM loop does
+    (one host check per product)."""
+
+    TAU = 2.0 ** -10
+
+    def __init__(self, op):
+        self.op = op
+        self.base = None
+        self.full = 0
+        self.calls = 0
+
+    def __call__(self, R):
+        self.calls += 1
+        R32 = R.to(torch.float32).contiguous()
+        if self.base is not None:
+            D = R32 - self.base[0]
+            moved = (D.abs().amax(dim=1) > self.TAU * R32.abs().amax(dim=1)).any()
+            if not bool(moved):
+                return self.base[1] + self.op.apply(D, hi_only=True)
+        Y = self.op.apply(R32)
+        self.base = (R32, Y)
+        self.full += 1
+        return Y
 
 
 def _neumann_terms(q, tol=1e-13, cap=40):
@@ -154,13 +184,28 @@ def model_deltaChi2_jackknife(density_func_, potential_func_, chi2_func_, params
     C = torch.eye(d, dtype=torch.float64, device='cuda') - torch.einsum('gdn,gen->gde', V, Pg)
     Cinv = torch.linalg.inv(C)
 
-    # ADMM (nnls.py:51-66), all replicates at once
+    # ADMM (nnls.py:51-66), all replicates at once; each Neumann term's input
+    # changes slowly from one iteration to the next, so its products are
+    # delta products between full ones (the last iteration's are full)
     z = torch.full((G, n), float(torch.sum(y_R)) / n, dtype=torch.float64, device='cuda')
     u = torch.zeros_like(z)
     alpha = 1.6
-    for _ in range(int(nnls_maxiter)):
+    terms = [_DeltaApply(op) for _ in range(T)]
+
+    def shifted_inverse_iter(R, dl, last):
+        if last:
+            for tm in terms:
+                tm.base = None
+        s = terms[0](R)
+        acc = s.clone()
+        for k in range(1, T):
+            s = terms[k](s)
+            acc += (dl ** k)[:, None] * s
+        return acc
+
+    for it in range(int(nnls_maxiter)):
         r = rho_g[:, None] * (z - u) - c_g
-        x = shifted_inverse(r, delta)
+        x = shifted_inverse_iter(r, delta, it == int(nnls_maxiter) - 1)
         tt = torch.einsum('gde,ge->gd', Cinv, torch.einsum('gdn,gn->gd', V, x))
         w = x + torch.einsum('gdn,gd->gn', Pg, tt)
         w_hat = alpha * w + (1.0 - alpha) * z
@@ -180,7 +225,9 @@ def model_deltaChi2_jackknife(density_func_, potential_func_, chi2_func_, params
                       e(y_h[3]), h4, sh[3])
     out = torch.std(chi2, unbiased=False)
     if return_aux:
-        return out, dict(chi2=chi2, weights=z, neumann_terms=T, rho=rho, delta=delta)
+        return out, dict(chi2=chi2, weights=z, neumann_terms=T, rho=rho, delta=delta,
+                         products=sum(tm.calls for tm in terms),
+                         full_products=sum(tm.full for tm in terms))
     return out
 
 

@@ -235,6 +235,59 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def bench_batched_grad(args):
+    """Forward + adjoint of M thetas, each a full-size model (gradients/s)."""
+    import torch
+    from schwarmax_amd import _lib, synth, grad as GR
+    cfg = synth.CONFIGS[args.config]
+    theta, dd, sched = make_workload(args.config, args.seed)
+    rng = np.random.default_rng(args.seed + 1)
+    M = cfg['n_models']
+    thetas = np.stack(synth.batched_thetas(M, rng))
+    B = int(dd['total_bins'])
+    P = int(dd['Rzphi_n_tot'])
+    T = cfg['T']
+
+    def step():
+        return GR.model_likelihood_and_grad_batch(thetas, dd, B, P, schedule=sched,
+                                                  nnls_maxiter=T, nnls_tol=args.nnls_tol)
+    for i in range(args.warmup):
+        t0 = time.time()
+        step()
+        torch.cuda.synchronize()
+        log(f'warmup {i}: {time.time() - t0:.1f}s')
+    torch.cuda.reset_peak_memory_stats()
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for i in range(args.steps):
+        v, g = step()
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    stages = _lib.timing_read()
+    _lib.timing_enable(False)
+    peak = torch.cuda.max_memory_allocated()
+    ok_v = int(torch.isfinite(v).sum())
+    ok_g = int(torch.isfinite(g).all(dim=1).sum())
+    log(f'batch of {M} gradients: {dt / args.steps:.1f} s; finite logL {ok_v}/{M}, '
+        f'finite gradients {ok_g}/{M}; peak HBM {peak / 2**30:.1f} GiB')
+    out = {
+        'metric': f'batched gradient evals/s ({M} theta, {int(dd["w0"].shape[0])} orbits each)',
+        'value': M * args.steps / dt, 'unit': 'gradient evals/s (logL and dlogL/dtheta)',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * dt / args.steps, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64/f32', 'data': 'synthetic',
+        'config': {'workload': args.config, 'models': M, 'orbits': int(dd['w0'].shape[0]),
+                   'realizations': sched['n_realizations'], 'bs3_attempts': sched['N_max'],
+                   'apertures': B, 'rzphi_cells': P,
+                   'bootstrap_rhs': dd['XY_standard_normal'].shape[0], 'admm_iters': T,
+                   'ckpt_every': 64},
+        'peak_hbm_bytes': int(peak), 'finite_logl': ok_v, 'finite_gradients': ok_g,
+        'stages_ms': {k: round(v_[0] / args.steps, 3) for k, v_ in stages.items()},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -269,6 +322,8 @@ def main():
     from schwarmax_amd import _lib, likelihood as L, nnls as NN
     from oracle import opcount
     from oracle import model as OM
+    if args.config.startswith('batched-grad'):
+        return bench_batched_grad(args)
     if args.config.startswith('batched'):
         return bench_batched(args)
     if args.config.startswith('jackknife'):

@@ -52,11 +52,15 @@ template <int N> struct Lift<Dual<N>> {
 };
 
 // chain rule helper: f(a) with f'(a) = g
+// f(a) with f'(a) = g.  A tangent that is exactly zero stays zero even where
+// g is infinite (sqrt or log at 0): that is the derivative reverse mode
+// (jax.grad) takes of sqrt(maximum(x, 0)) at a clamped x, where maximum's
+// select passes no cotangent, whereas 0 * inf would make the tangent NaN.
 template <int N> __device__ __forceinline__ Dual<N> dchain(const Dual<N>& a, double f, double g) {
   Dual<N> r;
   r.v = f;
 #pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = g * a.d[i];
+  for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] == 0.0) ? 0.0 : g * a.d[i];
   return r;
 }
 

@@ -317,3 +317,35 @@ def model_likelihood_and_grad(params, dict_data, num_Vbin, Rzphi_n_tot, sigma_am
                             nnls_maxiter, nnls_tol, ckpt_every)
     logl.backward()
     return logl.detach(), theta.grad.detach().clone()
+
+
+def model_likelihood_and_grad_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
+                                    sigma_amplify=1.0, schedule=None, nnls_maxiter=200,
+                                    nnls_tol=0.0, ckpt_every=64):
+    """(logL (M,), dlogL/dtheta (M, 12)) for the rows of thetas (M, 12).
+
+    Model m uses the first n_orbits[m] orbits of dict_data (default: all).
+    The models run one after another through model_likelihood_and_grad, each
+    freeing its slabs, checkpoints and NNLS workspace before the next (a
+    40,000-orbit model holds about 110 GB at its peak).
+    """
+    thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
+    M = thetas.shape[0]
+    n_all = np.asarray(dict_data['w0']).shape[0]
+    n_orbits = [n_all] * M if n_orbits is None else [int(v) for v in n_orbits]
+    if len(n_orbits) != M or min(n_orbits) < 1 or max(n_orbits) > n_all:
+        raise ValueError("n_orbits must hold M counts in [1, number of orbits in dict_data]")
+    logls = torch.empty(M, dtype=torch.float64, device='cuda')
+    grads = torch.empty((M, 12), dtype=torch.float64)
+    for m in range(M):
+        dm = dict(dict_data)
+        if n_orbits[m] < n_all:
+            for k in ('w0', 'jeans_normals', 'jitter_uniforms'):
+                if k in dm:
+                    dm[k] = np.asarray(dict_data[k])[:n_orbits[m]]
+        v, g = model_likelihood_and_grad(thetas[m], dm, num_Vbin, Rzphi_n_tot, sigma_amplify,
+                                         schedule, nnls_maxiter, nnls_tol, ckpt_every)
+        logls[m] = v
+        grads[m] = g
+        torch.cuda.empty_cache()
+    return logls, grads

@@ -266,3 +266,42 @@ def test_model_likelihood_gradient(galaxy):
     assert abs(plain - float(logl)) <= 1e-12 * abs(plain)
     err = np.abs(g - fd) / np.max(np.abs(fd))
     assert np.all(err < 1e-4), np.stack([g, fd, err])
+
+
+def test_gradient_batch_matches_per_model(galaxy):
+    """The batched forward + adjoint returns, per theta and orbit count, the
+    single-model logL and gradient."""
+    from schwarmax_amd import grad as GR, synth
+    theta, dd, sched = galaxy
+    rng = np.random.default_rng(5)
+    thetas = np.stack([theta] + synth.batched_thetas(1, rng))
+    n_all = np.asarray(dd['w0']).shape[0]
+    n_orbits = [n_all, n_all - 7]
+    v, g = GR.model_likelihood_and_grad_batch(thetas, dd, NAP, dd['Rzphi_n_tot'],
+                                              n_orbits=n_orbits, schedule=sched, nnls_maxiter=20,
+                                              ckpt_every=16)
+    for m in range(2):
+        dm = dict(dd)
+        for k in ('w0', 'jeans_normals', 'jitter_uniforms'):
+            dm[k] = np.asarray(dd[k])[:n_orbits[m]]
+        v1, g1 = GR.model_likelihood_and_grad(thetas[m], dm, NAP, dd['Rzphi_n_tot'],
+                                              schedule=sched, nnls_maxiter=20, ckpt_every=16)
+        assert float(v[m]) == float(v1), m
+        assert np.array_equal(g[m].numpy(), g1.numpy()), m
+
+
+def test_gradient_finite_where_jeans_moments_clamp():
+    """A parameter vector of the batched forward + adjoint workload whose
+    Jeans moments clamp (sqrt(max(x, 0)) at x <= 0) for some stars.  Forward
+    mode must give those stars zero tangents there, as jax.grad's reverse
+    mode does (maximum passes no cotangent to the clamped side); 0 * inf
+    would turn the whole gradient into NaN."""
+    import torch
+    import bench
+    from schwarmax_amd import grad as GR, synth
+    theta0, dd, sched = bench.make_workload('batched-grad-16', 0, n=8000)
+    thetas = np.stack(synth.batched_thetas(16, np.random.default_rng(1)))
+    v, g = GR.model_likelihood_and_grad(thetas[14], dd, int(dd['total_bins']),
+                                        int(dd['Rzphi_n_tot']), schedule=sched, nnls_maxiter=200)
+    assert np.isfinite(float(v))
+    assert torch.isfinite(g).all(), g

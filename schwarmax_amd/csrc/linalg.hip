@@ -926,20 +926,30 @@ __global__ __launch_bounds__(256) void build_ut_split_kernel(
 }
 
 // Lower-triangle tiles: row block I (128 rows), column block J (256 columns)
-// with 256 J <= 128 I, enumerated row block by row block.  Row block 2p holds
-// p + 1 tiles starting at p (p + 1), row block 2p + 1 p + 1 tiles starting at
-// (p + 1)^2.
-__device__ __forceinline__ void gram_tile(int64_t t, int64_t& I, int64_t& J) {
-  int64_t p = static_cast<int64_t>((sqrt(4.0 * static_cast<double>(t) + 1.0) - 1.0) * 0.5);
-  while (p > 0 && p * (p + 1) > t) --p;
-  while ((p + 1) * (p + 2) <= t) ++p;
-  if (t < (p + 1) * (p + 1)) {
-    I = 2 * p;
-    J = t - p * (p + 1);
-  } else {
-    I = 2 * p + 1;
-    J = t - (p + 1) * (p + 1);
-  }
+// with 256 J <= 128 I, i.e. I >= 2 J, enumerated column block by column
+// block: column block J holds the nrb - 2 J tiles I = 2 J .. nrb - 1,
+// starting at C(J) = J (nrb + 1 - J).
+__device__ __forceinline__ void gram_tile(int64_t q, int64_t nrb, int64_t& I, int64_t& J) {
+  const double b = static_cast<double>(nrb + 1);
+  int64_t j = static_cast<int64_t>((b - sqrt(b * b - 4.0 * static_cast<double>(q))) * 0.5);
+  if (j < 0) j = 0;
+  while (j > 0 && j * (nrb + 1 - j) > q) --j;
+  while ((j + 1) * (nrb - j) <= q) ++j;
+  J = j;
+  I = 2 * j + (q - j * (nrb + 1 - j));
+}
+
+// Workgroups go to the 8 XCDs round-robin by id; tile q of XCD x's
+// consecutive share runs on that XCD, so its co-resident workgroups work on
+// neighbouring tiles of one column block and read that block's B operand
+// through the same L2.  Returns false for the padding workgroups.
+__device__ __forceinline__ bool gram_tile_xcd(int64_t w, int64_t tiles, int64_t nrb, int64_t& I,
+                                              int64_t& J) {
+  const int64_t per = (tiles + 7) / 8;
+  const int64_t q = (w % 8) * per + w / 8;
+  if (q >= tiles) return false;
+  gram_tile(q, nrb, I, J);
+  return true;
 }
 
 static int64_t gram_tiles(int64_t gpad) {
@@ -948,19 +958,23 @@ static int64_t gram_tiles(int64_t gpad) {
   return q * (q + 1);
 }
 
+static int64_t gram_grid(int64_t gpad) { return 8 * ((gram_tiles(gpad) + 7) / 8); }
+
 // One (128 x 256) tile of Q per workgroup (4 waves of 128 x 64), 32-index
 // chunks staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep step; writes
 // Q[i][j] for j <= i < n.
 __global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
-    int64_t n, int64_t mpad, const _Float16* __restrict__ Uh, const _Float16* __restrict__ Ul,
-    const float* __restrict__ us, double* __restrict__ Q) {
+    int64_t n, int64_t gpad, int64_t mpad, const _Float16* __restrict__ Uh,
+    const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q) {
   constexpr int P = AG_KP + 8;
   __shared__ __align__(16) _Float16 Bh[2 * GR_TM][P];
   __shared__ __align__(16) _Float16 Bl[2 * GR_TM][P];
   __shared__ __align__(16) _Float16 Ah[GR_TM][P];
   __shared__ __align__(16) _Float16 Al[GR_TM][P];
+  const int64_t nrb = gpad / GR_TM;
+  const int64_t nq = (nrb / 2) * (nrb / 2 + 1);
   int64_t I, J;
-  gram_tile(blockIdx.x, I, J);
+  if (!gram_tile_xcd(blockIdx.x, nq, nrb, I, J)) return;   // whole workgroup
   const int64_t i0 = I * GR_TM, j0 = J * 2 * GR_TM;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 5, c = lane & 31;
@@ -1274,8 +1288,8 @@ static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64
   hipLaunchKernelGGL(build_ut_split_kernel, dim3((unsigned)w.gpad), dim3(256), 0, st, lib, ld_lib,
                      n, m, ldu, w.mpad, row_scale, w.Ut, w.Uh, w.Ul, w.us);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_tiles(w.gpad)), dim3(256), 0, st, n,
-                     w.mpad, w.Uh, w.Ul, w.us, w.Q64);
+  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(256), 0, st, n,
+                     w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
                      lambda_reg / static_cast<double>(n), w.rho64, w.rho);

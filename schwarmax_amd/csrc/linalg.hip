@@ -1069,10 +1069,15 @@ constexpr int INV_BASE = 128;
 //   a_kk <- -1/a_kk,  a_ik <- a_ik/a_kk,  a_kj <- a_kj/a_kk,
 //   a_ij <- a_ij - a_ik a_kj / a_kk.
 // Sweeping all pivots leaves -A^-1; the full block is written back negated.
-// Pivots run in groups of 8 with the position inside the group unrolled, so
-// every register index is static.
+// Every element takes the rank-1 update, then the threads holding the pivot
+// row or column overwrite those (so no wave runs two update paths).  The
+// published column alternates between two LDS buffers: a thread writing
+// pivot k's column has passed barrier k-1, which every reader of pivot k-2
+// (the same buffer) reached only after its reads, so one barrier per pivot
+// suffices.  Pivots run in groups of 8 with the position inside the group
+// unrolled, so every register index is static.
 __global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
-  __shared__ double col[INV_BASE];
+  __shared__ double col[2][INV_BASE];
   const int t = threadIdx.x;
   const int bi = t >> 4, bj = t & 15;
   double a[8][8];
@@ -1089,32 +1094,31 @@ __global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, i
 #pragma unroll
     for (int kc = 0; kc < 8; ++kc) {
       const int k = 8 * kb + kc;
+      double* cb = col[kc & 1];
       if (bj == kb) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) col[8 * bi + r] = a[r][kc];
+        for (int r = 0; r < 8; ++r) cb[8 * bi + r] = a[r][kc];
       }
       __syncthreads();
-      const double f = 1.0 / col[k];
+      const double f = 1.0 / cb[k];
       double gr[8], gc[8];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) gr[r] = col[8 * bi + r] * f;
+      for (int r = 0; r < 8; ++r) gr[r] = cb[8 * bi + r] * f;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) gc[c] = col[8 * bj + c];
-      if (bi != kb && bj != kb) {
+      for (int c = 0; c < 8; ++c) gc[c] = cb[8 * bj + c];
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < 8; ++r)
 #pragma unroll
-          for (int c = 0; c < 8; ++c) a[r][c] -= gr[r] * gc[c];
-      } else {
+        for (int c = 0; c < 8; ++c) a[r][c] -= gr[r] * gc[c];
+      if (bi == kb) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
-#pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            const bool rk = (bi == kb) && (r == kc), ck = (bj == kb) && (c == kc);
-            a[r][c] = rk ? (ck ? -f : gc[c] * f) : (ck ? gr[r] : a[r][c] - gr[r] * gc[c]);
-          }
+        for (int c = 0; c < 8; ++c) a[kc][c] = gc[c] * f;
       }
-      __syncthreads();
+      if (bj == kb) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) a[r][kc] = gr[r];
+        if (bi == kb) a[kc][kc] = -f;
+      }
     }
   }
 #pragma unroll
@@ -1220,6 +1224,17 @@ static int blas_dsyrkx(hipStream_t st, bool trans, int64_t n, int64_t k, double 
 //   S  -= T A21^T   (lower)            dsyrkx(T): (A21^T)^T T^T ... = A21 T^T
 //   A21^T = -T^T S^-1                  dgemm(N, N, n1, n2, n2)
 //   X  -= T^T A21   (lower)            dsyrkx(N): A21^T (T^T)^T = A21^T T
+// Below INV_SYRKX_MIN rows the two symmetric updates run as full dgemm
+// products instead: twice the flops, but one launch where rocBLAS syrkx runs
+// a diagonal-block kernel plus several off-diagonal products.  Measured on
+// the 20,000-row inverse, the full products are as fast as syrkx at the
+// 10,000-row nodes and much faster below (2,500-row inverse 5.7 -> 2.9 ms);
+// syrkx keeps the 20,000-row node, where its halved flops win.  The full
+// product also rewrites the block's upper triangle, which nothing reads
+// before mirror_lower restores it.
+#ifndef INV_SYRKX_MIN
+#define INV_SYRKX_MIN 16384
+#endif
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
     hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
@@ -1235,12 +1250,16 @@ static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double
   double* deeper = work + n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
   mirror_lower(st, A11, n1, lda);
+  const bool sym = n >= INV_SYRKX_MIN;
   if (int e = blas_dgemm(st, false, false, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, T, n1)) return e;
-  if (int e = blas_dsyrkx(st, true, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda)) return e;
+  if (int e = sym ? blas_dsyrkx(st, true, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda)
+                  : blas_dgemm(st, true, false, n2, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda))
+    return e;
   if (int e = inv_spd_rec(st, A22, n2, lda, deeper)) return e;
   mirror_lower(st, A22, n2, lda);
   if (int e = blas_dgemm(st, false, false, n1, n2, n2, -1.0, T, n1, A22, lda, 0.0, A21, lda)) return e;
-  return blas_dsyrkx(st, false, n1, n2, -1.0, A21, lda, T, n1, 1.0, A11, lda);
+  return sym ? blas_dsyrkx(st, false, n1, n2, -1.0, A21, lda, T, n1, 1.0, A11, lda)
+             : blas_dgemm(st, false, true, n1, n1, n2, -1.0, A21, lda, T, n1, 1.0, A11, lda);
 }
 
 // Full inverse of the SPD matrix whose lower triangle is given.

@@ -51,3 +51,29 @@ def test_spd_inverse(T, n):
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err < 1e-13 * np.linalg.cond(Q), err
     assert np.abs(got - got.T).max() <= 1e-12 * np.abs(got).max()
+
+
+def test_spd_inverse_large(T):
+    """A 16,500-row inverse, whose top node takes rocBLAS syrkx and whose
+    lower nodes take full dgemm updates: residual A X - I and symmetry on the
+    device (the CPU inverse would take minutes)."""
+    from schwarmax_amd import _lib
+    n = 16500
+    g = T.Generator(device='cuda')
+    g.manual_seed(7)
+    X = T.rand((n + n // 4, n), dtype=T.float64, device='cuda', generator=g)
+    X *= (T.rand(X.shape, device='cuda', generator=g) < 0.2)
+    X[:, : n // 3] *= 30.0
+    Q = X.T @ X
+    del X
+    Q += (Q.diagonal().sum() / n) * T.eye(n, dtype=T.float64, device='cuda')
+    A = Q.clone()
+    work = T.empty(n * n + n, dtype=T.float64, device='cuda')
+    _lib.check(_lib.lib().smx_spd_inverse_f64(_lib.ptr(A), n, _lib.ptr(work), _lib.stream_ptr()), 'inv')
+    del work
+    R = Q @ A
+    R.diagonal().sub_(1.0)
+    res = R.abs().max().item()
+    asym = (A - A.T).abs().max().item() / A.abs().max().item()
+    assert res < 1e-10, res
+    assert asym <= 1e-12, asym

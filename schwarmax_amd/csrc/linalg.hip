@@ -473,6 +473,14 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
   }
 }
 
+// Bijection of [0, G): block b, dispatched to XCD b % 8, gets the index that
+// makes each XCD's blocks one contiguous run (the first G % 8 XCDs take
+// G / 8 + 1 blocks, the rest G / 8).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
+  const int64_t q = G / 8, r = G % 8, x = b % 8, slot = b / 8;
+  return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // One ADMM product: the full one on R (scales from rmax), or the delta one on
 // D (scales from dmax), as admm_full_product decides.  D == nullptr: always
 // full (force must then be 1).
@@ -491,8 +499,11 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
   // 1-D grid, row block of R fastest (the Kp/128 row blocks that share a tile
   // of M run back to back and share it through the caches), then column
   // block, then split; with one row block this is column block, then split.
+  // Workgroups are dispatched to the 8 XCDs round-robin; xcd_block gives each
+  // XCD a contiguous run of this order, so an XCD's L2 holds the R (or D)
+  // chunk ranges of one or two splits rather than all of R.
   const int64_t nrb = Kp / AG_KR, ncb = npad / AG_BN;
-  const int64_t bid = blockIdx.x;
+  const int64_t bid = xcd_block(blockIdx.x, gridDim.x);
   const int64_t k0 = (bid % nrb) * AG_KR;
   const int64_t j0 = ((bid / nrb) % ncb) * AG_BN;
   const int split = (int)(bid / (nrb * ncb));
@@ -516,8 +527,11 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
     stg.row_scales(dmax, k0);
     admm_gemm_loop<KC, false>(Kp, npad, k0, j0, ch0, ch1, D, Mh, Ml, stg, Bh, Bl, Ah, Al, acc);
   }
-  // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
+  // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4,
+  // column c.  Rows K..Kp-1 (zero padding of R) are not stored: every reader
+  // of the partials sums rows k < K only.
   float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / 4);
+  const int64_t rows = K - k0;
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -525,7 +539,7 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        out[(int64_t)row * npad + cb * 32 + c] = acc[rb][cb][r];
+        if (row < rows) out[(int64_t)row * npad + cb * 32 + c] = acc[rb][cb][r];
       }
 }
 

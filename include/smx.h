@@ -255,8 +255,9 @@ int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* wor
 
 /* ---- ADMM product tuning (no reference counterpart: the knob the
  * autotuner in schwarmax_amd/tune.py sets) ------------------------------- */
-/* Workgroups per CU that the ADMM product's split-K rule targets (1 ..
- * 4; 0 restores the default, 2).  Process-wide, atomic; applies to solves
+/* 256-thread workgroups per CU that the ADMM product's split-K rule targets
+ * (1 .. 8; 0 restores the default, 2; the rule keeps at least one workgroup
+ * per CU).  Process-wide, atomic; applies to solves
  * started afterwards (workspaces are sized for the largest setting). */
 int smx_admm_set_occupancy(int32_t workgroups_per_cu);
 int smx_admm_get_occupancy(int32_t* workgroups_per_cu);

@@ -826,21 +826,37 @@ struct NnlsWs {
 };
 
 // Splits of the ADMM product's reduction: enough workgroups for `occ` on each
-// of the 256 CUs.  occ is the product's tuning knob (smx_admm_set_occupancy,
-// set by the autotuner; default AG_OCC, the workgroups per CU that the
-// kernel's LDS and registers allow at once).  Workspaces size the partial
-// buffer for occ = AG_OCC_MAX, so a later setting never overruns one.
-constexpr int AG_OCC_MAX = 4;
+// of the 256 CUs, and at least one workgroup per CU.  occ is the product's
+// tuning knob (smx_admm_set_occupancy, set by the autotuner; default AG_OCC,
+// the workgroups per CU that the kernel's LDS and registers allow at once;
+// larger values queue further rounds of workgroups).  Workspaces size the
+// partial buffer for occ = AG_OCC_MAX, and S grows with occ, so a later
+// setting never overruns one.
+constexpr int AG_OCC_MAX = 8;
 static std::atomic<int> g_admm_occ{AG_OCC};
 
 static int admm_splits_for(int64_t npad, int64_t Kp, int occ) {
-  const int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);
-  int64_t S = (256 * (int64_t)occ) / (tiles > 0 ? tiles : 1);
-  if (S < 1) S = 1;
+  int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);   // workgroups of one split
+  if (tiles < 1) tiles = 1;
+  int64_t S = (256 * (int64_t)occ) / tiles;
+  const int64_t fill = (256 + tiles - 1) / tiles;
+  if (S < fill) S = fill;
   if (S > 16) S = 16;
   const int64_t nch = npad / AG_KP;
   if (S > nch) S = nch;
   return static_cast<int>(S);
+}
+
+// One ADMM product launch (admm_gemm_kernel) over S splits of the reduction.
+static void launch_admm_product(hipStream_t st, int64_t K, int64_t Kp, int64_t npad, int S,
+                                const float* R, const unsigned* rmax, const float* D,
+                                const unsigned* dmax, int force, const _Float16* Mh,
+                                const _Float16* Ml, float* part) {
+  const int64_t nch = npad / AG_KP;
+  const int cps = static_cast<int>((nch + S - 1) / S);
+  const dim3 grid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
+  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), grid, dim3(256), 0, st, K, Kp, npad, cps, R, rmax,
+                     D, dmax, force, Mh, Ml, part);
 }
 
 static int admm_splits(int64_t npad, int64_t Kp) {
@@ -1387,9 +1403,6 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   const int check = 25;
   std::vector<unsigned> dl(K), zm(K);
   int done = 0;
-  const int64_t nch = npad / AG_KP;
-  const int cps = static_cast<int>((nch + S - 1) / S);
-  dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM, st);
   for (int it = 0; it < maxiter; ++it) {
     const bool chk = tol > 0.0 && ((it + 1) % check == 0);
@@ -1412,8 +1425,8 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     const int force = (it == 0 || it == maxiter - 1) ? 1 : 0;
     const int first = it == 0 ? 1 : 0;
     stage_begin(ST_ADMM_PRODUCT, st);
-    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
-                       ib.rcur, w.D, ib.dcur, force, w.Mh, w.Ml, w.part);
+    launch_admm_product(st, K, Kp, npad, S, w.Rp, ib.rcur, w.D, ib.dcur, force, w.Mh, w.Ml,
+                        w.part);
     SMX_LAUNCHED();
     stage_end(ST_ADMM_PRODUCT, st, 1);
     if (n % 4 == 0)
@@ -1540,16 +1553,13 @@ int smx_admm_product_time(int64_t n, int64_t m, int64_t K, void* workspace, int3
   hipStream_t st = as_stream(stream);
   NnlsWs w = carve(workspace, n, m, K);
   const int64_t npad = w.npad, Kp = w.Kp;
-  const int64_t nch = npad / AG_KP;
-  const int cps = static_cast<int>((nch + w.S - 1) / w.S);
-  dim3 ggrid((unsigned)((npad / AG_BN) * w.S * (Kp / AG_KR)));
   hipEvent_t e0, e1;
   SMX_HIP(hipEventCreate(&e0));
   SMX_HIP(hipEventCreate(&e1));
   SMX_HIP(hipEventRecord(e0, st));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
-                       w.rmax, w.D, w.rmax + 3 * Kp, full ? 1 : -1, w.Mh, w.Ml, w.part);
+    launch_admm_product(st, K, Kp, npad, w.S, w.Rp, w.rmax, w.D, w.rmax + 3 * Kp, full ? 1 : -1,
+                        w.Mh, w.Ml, w.part);
   SMX_LAUNCHED();
   SMX_HIP(hipEventRecord(e1, st));
   SMX_HIP(hipEventSynchronize(e1));
@@ -1590,17 +1600,14 @@ static int operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, 
   NnlsWs w = carve(workspace, n, m, K_max);
   const int64_t npad = w.npad, Kp = align_up(K, AG_KR);
   const int S = w.S;
-  const int64_t nch = npad / AG_KP;
-  const int cps = static_cast<int>((nch + S - 1) / S);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp, 256)), dim3(256), 0, st, w.rmax, Kp, 0u);
   hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)(npad / 256), (unsigned)Kp), dim3(256), 0, st,
                      R, K, n, Kp, w.Rp, w.rmax);
   SMX_LAUNCHED();
-  dim3 ggrid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   stage_begin(ST_ADMM_PRODUCT, st);
   // the padded rows serve as R (full) or as D (delta), with the same row maxima
-  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), ggrid, dim3(256), 0, st, K, Kp, npad, cps, w.Rp,
-                     w.rmax, w.Rp, w.rmax, hi_only ? -1 : 1, w.Mh, w.Ml, w.part);
+  launch_admm_product(st, K, Kp, npad, S, w.Rp, w.rmax, w.Rp, w.rmax, hi_only ? -1 : 1, w.Mh, w.Ml,
+                      w.part);
   SMX_LAUNCHED();
   stage_end(ST_ADMM_PRODUCT, st, 1);
   hipLaunchKernelGGL(apply_out_kernel, dim3(blocks_for(K * n, 256)), dim3(256), 0, st, w.part, S,

@@ -321,13 +321,15 @@ def model_likelihood_and_grad(params, dict_data, num_Vbin, Rzphi_n_tot, sigma_am
 
 def model_likelihood_and_grad_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
                                     sigma_amplify=1.0, schedule=None, nnls_maxiter=200,
-                                    nnls_tol=0.0, ckpt_every=64):
+                                    nnls_tol=0.0, ckpt_every=64, autotune=True):
     """(logL (M,), dlogL/dtheta (M, 12)) for the rows of thetas (M, 12).
 
     Model m uses the first n_orbits[m] orbits of dict_data (default: all).
     The models run one after another through model_likelihood_and_grad, each
     freeing its slabs, checkpoints and NNLS workspace before the next (a
     40,000-orbit model holds about 110 GB at its peak).
+    autotune: pick the ADMM product's occupancy for the batch's median shape
+    first (``tune.ensure_tuned``, once per process and shape bucket).
     """
     thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
     M = thetas.shape[0]
@@ -335,6 +337,10 @@ def model_likelihood_and_grad_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_
     n_orbits = [n_all] * M if n_orbits is None else [int(v) for v in n_orbits]
     if len(n_orbits) != M or min(n_orbits) < 1 or max(n_orbits) > n_all:
         raise ValueError("n_orbits must hold M counts in [1, number of orbits in dict_data]")
+    if autotune:
+        from . import tune
+        K = np.asarray(dict_data['XY_standard_normal']).shape[0]
+        tune.ensure_tuned(int(np.median(n_orbits)), int(Rzphi_n_tot) + 5 * int(num_Vbin), K)
     logls = torch.empty(M, dtype=torch.float64, device='cuda')
     grads = torch.empty((M, 12), dtype=torch.float64)
     for m in range(M):

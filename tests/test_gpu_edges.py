@@ -53,10 +53,11 @@ def test_zero_iterations_return_the_start(cuda):
     assert abs(got[0, 0] - np.sum(y_R) / 130) <= 1e-14 * abs(np.sum(y_R) / 130)
 
 
-@pytest.mark.parametrize("n,K", [(131, 5), (301, 150)])
+@pytest.mark.parametrize("n,K", [(131, 5), (301, 150), (257, 256), (200, 300)])
 def test_ragged_shapes(cuda, n, K):
     """n not a multiple of 4 (the update kernel's scalar path) or of 256 (the
-    product's column blocks); K above 128 (two row blocks of R)."""
+    product's column blocks); K above 128: two row blocks of R (the second
+    partly and wholly filled) and three."""
     from schwarmax_amd import nnls as GN
     from oracle import nnls as ON
     from test_gpu_nnls import _problem

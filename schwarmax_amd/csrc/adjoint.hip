@@ -423,18 +423,28 @@ __global__ __launch_bounds__(64) void library_vjp_kernel(
 // ---------------------------------------------------------------------------
 // Field Jacobians of the initial conditions and the density target.
 // ---------------------------------------------------------------------------
-template <int NJ>
-__global__ __launch_bounds__(256) void ic_jac_kernel(smx_potential p, const double* __restrict__ w0,
-                                                     const double* __restrict__ normals,
-                                                     const double* __restrict__ uniforms, int64_t n,
-                                                     int r, int r_total, double N_step, int f0,
-                                                     double* __restrict__ dwb,
-                                                     double* __restrict__ ddt0) {
-  const int lane = threadIdx.x & 63;
-  const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (i >= n) return;
+// One group of NJ fields (F0 .. F0 + NJ - 1) of the initial conditions'
+// Jacobian, from the wave's three pressure integrals in gw (value, then the
+// PF_COUNT field derivatives, each).  F0 is a constant, so the compiler drops
+// the tangent arithmetic of the unseeded fields; out of line, so that the
+// passes do not share (and keep live) their common primal values.
+constexpr int IC_GW = PF_COUNT + 1;
+
+template <int NJ, int F0>
+__device__ __noinline__ void ic_jac_pass(const smx_potential& p, const double* __restrict__ w0,
+                                         const double* __restrict__ normals,
+                                         const double* __restrict__ uniforms, int64_t i, int lane,
+                                         int r, int r_total, double N_step, const double* gw,
+                                         double* __restrict__ dwb, double* __restrict__ ddt0) {
+  Dual<NJ> I[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    I[q] = dconst<NJ>(gw[q * IC_GW]);
+#pragma unroll
+    for (int k = 0; k < NJ; ++k) I[q].d[k] = (F0 + k < PF_COUNT) ? gw[q * IC_GW + 1 + F0 + k] : 0.0;
+  }
   StarIC<Dual<NJ>> o;
-  star_ic<Dual<NJ>>(p, fields_seeded<NJ>(p, f0), w0, normals, i, lane, o);
+  star_ic_finish<Dual<NJ>>(p, fields_seeded<NJ>(p, F0), w0, normals, i, I[0], I[1], I[2], o);
   if (lane != 0) return;
   for (int k = 0; k < r; ++k) {
     const int64_t q = i * r + k;
@@ -442,15 +452,54 @@ __global__ __launch_bounds__(256) void ic_jac_kernel(smx_potential p, const doub
       const double u = uniforms[(i * r_total + k) * 6 + j] - 0.5;
 #pragma unroll
       for (int t = 0; t < NJ; ++t) {
-        if (f0 + t >= PF_COUNT) continue;
+        if (F0 + t >= PF_COUNT) continue;
         const double ds = (j < 3) ? 0.0 : o.vsc.d[t];
-        dwb[(6 * q + j) * PF_COUNT + f0 + t] = o.wn[j].d[t] + u * ds;
+        dwb[(6 * q + j) * PF_COUNT + F0 + t] = o.wn[j].d[t] + u * ds;
       }
     }
 #pragma unroll
     for (int t = 0; t < NJ; ++t)
-      if (f0 + t < PF_COUNT) ddt0[q * PF_COUNT + f0 + t] = o.T_orb.d[t] / N_step;
+      if (F0 + t < PF_COUNT) ddt0[q * PF_COUNT + F0 + t] = o.T_orb.d[t] / N_step;
   }
+}
+
+// The pressure integrals and their field derivatives come from one pass
+// (pressure_part_grad); the rest of the initial conditions runs once per
+// group of NJ fields on Dual<NJ> (ic_jac_pass).  The wave's three integrals
+// wait in LDS (lane 0 writes them, the wave's lanes read them back in
+// order), so that they hold no registers during those passes.
+template <int NJ>
+__global__ __launch_bounds__(256) void ic_jac_kernel(smx_potential p, const double* __restrict__ w0,
+                                                     const double* __restrict__ normals,
+                                                     const double* __restrict__ uniforms, int64_t n,
+                                                     int r, int r_total, double N_step,
+                                                     double* __restrict__ dwb,
+                                                     double* __restrict__ ddt0) {
+  static_assert(NJ == 4 && PF_COUNT <= 12, "three passes of four fields");
+  __shared__ double gsh[4][3 * IC_GW];
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  double* gw = gsh[threadIdx.x >> 6];
+  {
+    const double xs = w0[3 * i], ys = w0[3 * i + 1], zs = w0[3 * i + 2];
+    const double Rs = sqrt(xs * xs + ys * ys);
+    const double zlo = fabs(zs);
+    const FieldGrad I[3] = {
+        wave_sum(pressure_part_grad(p, xs, ys, zlo, JEANS_NPTS, lane)),
+        wave_sum(pressure_part_grad(p, Rs + JEANS_DR, 0.0, zlo, JEANS_NPTS, lane)),
+        wave_sum(pressure_part_grad(p, Rs - JEANS_DR, 0.0, zlo, JEANS_NPTS, lane))};
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        gw[q * IC_GW] = I[q].v;
+#pragma unroll
+        for (int k = 0; k < PF_COUNT; ++k) gw[q * IC_GW + 1 + k] = I[q].d[k];
+      }
+  }
+  ic_jac_pass<NJ, 0>(p, w0, normals, uniforms, i, lane, r, r_total, N_step, gw, dwb, ddt0);
+  ic_jac_pass<NJ, NJ>(p, w0, normals, uniforms, i, lane, r, r_total, N_step, gw, dwb, ddt0);
+  ic_jac_pass<NJ, 2 * NJ>(p, w0, normals, uniforms, i, lane, r, r_total, N_step, gw, dwb, ddt0);
 }
 
 template <int NJ>
@@ -532,13 +581,11 @@ int smx_initial_conditions_jac(const smx_potential* p, const double* w0, const d
   hipStream_t st = as_stream(stream);
   const int waves_per_block = 4;
   stage_begin(ST_JACOBIAN, st);
-  for (int f0 = 0; f0 < PF_COUNT; f0 += JAC_SLOTS) {
-    hipLaunchKernelGGL(ic_jac_kernel<JAC_SLOTS>, dim3(blocks_for(n, waves_per_block)),
-                       dim3(64 * waves_per_block), 0, st, *p, w0, normals, uniforms, n, (int)r,
-                       (int)r_total, N_step, f0, wb_jac, dt0_jac);
-    SMX_LAUNCHED();
-  }
-  stage_end(ST_JACOBIAN, st, 3);
+  hipLaunchKernelGGL(ic_jac_kernel<JAC_SLOTS>, dim3(blocks_for(n, waves_per_block)),
+                     dim3(64 * waves_per_block), 0, st, *p, w0, normals, uniforms, n, (int)r,
+                     (int)r_total, N_step, wb_jac, dt0_jac);
+  SMX_LAUNCHED();
+  stage_end(ST_JACOBIAN, st, 1);
   return 0;
 }
 

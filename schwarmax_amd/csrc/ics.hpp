@@ -83,18 +83,18 @@ struct StarIC {
   S T_orb;      // orbital time scale
 };
 
-// One 64-lane wave per star; every lane returns the same values.
+constexpr int JEANS_NPTS = 1000;     // trapezoid points (jeans.py:37, 57)
+constexpr double JEANS_DR = 5e-3;    // radial finite-difference step (jeans.py:60-61)
+
+// The rest of star_ic, from the three pressure integrals at (x, y), (R + dR,
+// 0) and (R - dR, 0).
 template <class S>
-__device__ void star_ic(const smx_potential& p, const PotFields<S>& f, const double* w0,
-                        const double* normals, int64_t i, int lane, StarIC<S>& o) {
+__device__ void star_ic_finish(const smx_potential& p, const PotFields<S>& f, const double* w0,
+                               const double* normals, int64_t i, const S& I0, const S& Ip,
+                               const S& Im, StarIC<S>& o) {
   const double xs = w0[3 * i], ys = w0[3 * i + 1], zs = w0[3 * i + 2];
   const double Rs = sqrt(xs * xs + ys * ys);
-  const double zlo = fabs(zs);
-  const int npts = 1000;
-  const double dR = 5e-3;
-  const S I0 = wave_sum(pressure_part_t(p, f, xs, ys, zlo, npts, lane));
-  const S Ip = wave_sum(pressure_part_t(p, f, Rs + dR, 0.0, zlo, npts, lane));
-  const S Im = wave_sum(pressure_part_t(p, f, Rs - dR, 0.0, zlo, npts, lane));
+  const double dR = JEANS_DR;
   const double b = 1.0;   // anisotropy_b (jeans.py:86)
   const S nu = rho_total_t(p, f, Lift<S>::c(xs), Lift<S>::c(ys), Lift<S>::c(zs));
   const S sz2 = smax((1.0 / nu) * I0, 0.0);
@@ -135,6 +135,123 @@ __device__ void star_ic(const smx_potential& p, const PotFields<S>& f, const dou
   const S om = sqrt(smax(fabs(dd) / Rsafe, 1e-20));
   o.T_orb = (2.0 * kPi) / om;
   o.vsc = smin(smax(0.1 * vc, 1.0), 15.0);
+}
+
+// One 64-lane wave per star; every lane returns the same values.
+template <class S>
+__device__ void star_ic(const smx_potential& p, const PotFields<S>& f, const double* w0,
+                        const double* normals, int64_t i, int lane, StarIC<S>& o) {
+  const double xs = w0[3 * i], ys = w0[3 * i + 1], zs = w0[3 * i + 2];
+  const double Rs = sqrt(xs * xs + ys * ys);
+  const double zlo = fabs(zs);
+  const S I0 = wave_sum(pressure_part_t(p, f, xs, ys, zlo, JEANS_NPTS, lane));
+  const S Ip = wave_sum(pressure_part_t(p, f, Rs + JEANS_DR, 0.0, zlo, JEANS_NPTS, lane));
+  const S Im = wave_sum(pressure_part_t(p, f, Rs - JEANS_DR, 0.0, zlo, JEANS_NPTS, lane));
+  star_ic_finish(p, f, w0, normals, i, I0, Ip, Im, o);
+}
+
+// A pressure integral with its derivatives with respect to every potential
+// field (PotField order).
+struct FieldGrad {
+  double v;
+  double d[PF_COUNT];
+};
+
+__device__ __forceinline__ FieldGrad wave_sum(FieldGrad g) {
+  g.v = wave_sum(g.v);
+#pragma unroll
+  for (int k = 0; k < PF_COUNT; ++k) g.d[k] = wave_sum(g.d[k]);
+  return g;
+}
+
+// Density and central-difference dPhi/dz of one component at one point,
+// with tangents on that component's fields (kept out of line: inlined
+// together, the components' dual evaluations exceed the register file).
+__device__ __noinline__ void t3_point(const smx_potential& p, double x, double y, double z, double d,
+                                      Dual<4>& rho, Dual<4>& dz) {
+  const Dual<4> M = dseed<4>(p.t3_M, 0), L = dseed<4>(p.t3_L, 1), a = dseed<4>(p.t3_a, 2),
+                b = dseed<4>(p.t3_b, 3);
+  const Dual<4> X = dconst<4>(x), Y = dconst<4>(y);
+  rho = t3_rho_t(p, M, L, a, b, X, Y, dconst<4>(z));
+  dz = ((-p.G) * t3_ph_t(p, M, L, a, b, X, Y, dconst<4>(z + d)) -
+        (-p.G) * t3_ph_t(p, M, L, a, b, X, Y, dconst<4>(z - d))) /
+       (2.0 * d);
+}
+
+__device__ __noinline__ void mn_point(const smx_potential& p, double x, double y, double z, double d,
+                                      Dual<3>& rho, Dual<3>& dz) {
+  const Dual<3> X = dconst<3>(x), Y = dconst<3>(y);
+  rho = mn_rho_t(p, dseed<3>(p.mn_rho0, 0), dseed<3>(p.mn_Rs, 1), dseed<3>(p.mn_Hs, 2), X, Y,
+                 dconst<3>(z));
+  const Dual<3> m = dseed<3>(p.mn_mGM, 0), R = dseed<3>(p.mn_Rs, 1), H = dseed<3>(p.mn_Hs, 2);
+  dz = (mn_phi_t(p, m, R, H, X, Y, dconst<3>(z + d)) - mn_phi_t(p, m, R, H, X, Y, dconst<3>(z - d))) /
+       (2.0 * d);
+}
+
+__device__ __noinline__ void nfw_v4_point(const smx_potential& p, double x, double y, double z,
+                                          double d, Dual<2>& hz, Dual<1>& vrho, Dual<1>& vz) {
+  const Dual<2> m = dseed<2>(p.nfw_mGM, 0), R = dseed<2>(p.nfw_Rs, 1);
+  hz = (nfw_phi_t(p, m, R, dconst<2>(x), dconst<2>(y), dconst<2>(z + d)) -
+        nfw_phi_t(p, m, R, dconst<2>(x), dconst<2>(y), dconst<2>(z - d))) /
+       (2.0 * d);
+  const Dual<1> M = dseed<1>(p.v4_M, 0), X = dconst<1>(x), Y = dconst<1>(y);
+  vrho = v4_rho_t(p, M, X, Y, dconst<1>(z));
+  vz = ((-p.G) * v4_ph_t(p, M, X, Y, dconst<1>(z + d)) - (-p.G) * v4_ph_t(p, M, X, Y, dconst<1>(z - d))) /
+       (2.0 * d);
+}
+
+// this lane's share of pressure_part_t and of all its field derivatives in
+// one pass.  Every field belongs to one component (NFW: mGM, Rs; MN: mGM,
+// rho0, Rs, Hs; T3: M, L, a, b; V4: M), rho and Phi are sums over
+// components, so per point each component's density and potential are
+// evaluated once with tangents on its own fields only, and the integrand
+// rho dPhi/dz is differentiated by the product rule.
+__device__ FieldGrad pressure_part_grad(const smx_potential& p, double x, double y, double zlo,
+                                        int npts, int lane) {
+  const double hi = 10.0;
+  const double step = (hi - zlo) / static_cast<double>(npts - 1);
+  const double d = 5e-3;   // dphi_dz_t's step
+  FieldGrad acc;
+  acc.v = 0.0;
+#pragma unroll
+  for (int k = 0; k < PF_COUNT; ++k) acc.d[k] = 0.0;
+#pragma unroll 1
+  for (int i = lane; i < npts; i += 64) {
+    const double zi = lin_pt(zlo, hi, step, i, npts);
+    Dual<4> tr, tz;
+    Dual<3> mr, mz;
+    Dual<2> hz;
+    Dual<1> vr, vz;
+    t3_point(p, x, y, zi, d, tr, tz);
+    mn_point(p, x, y, zi, d, mr, mz);
+    nfw_v4_point(p, x, y, zi, d, hz, vr, vz);
+    const double rho = (mr.v + tr.v) + vr.v;
+    const double G = ((hz.v + mz.v) + tz.v) + vz.v;
+    double w = 0.0;
+    if (i > 0) w += zi - lin_pt(zlo, hi, step, i - 1, npts);
+    if (i < npts - 1) w += lin_pt(zlo, hi, step, i + 1, npts) - zi;
+    const double c = 0.5 * w;
+    acc.v += c * (rho * G);
+    acc.d[PF_NFW_MGM] += c * (rho * hz.d[0]);
+    acc.d[PF_NFW_RS] += c * (rho * hz.d[1]);
+    acc.d[PF_MN_MGM] += c * (rho * mz.d[0]);
+    acc.d[PF_MN_RHO0] += c * (mr.d[0] * G);
+    acc.d[PF_MN_RS] += c * (mr.d[1] * G + rho * mz.d[1]);
+    acc.d[PF_MN_HS] += c * (mr.d[2] * G + rho * mz.d[2]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc.d[PF_T3_M + k] += c * (tr.d[k] * G + rho * tz.d[k]);
+    acc.d[PF_V4_M] += c * (vr.d[0] * G + rho * vz.d[0]);
+  }
+  return acc;
+}
+
+// fields f0 .. f0 + N - 1 of a FieldGrad as a Dual<N> (beyond PF_COUNT: 0)
+template <int N>
+__device__ __forceinline__ Dual<N> grad_slice(const FieldGrad& g, int f0) {
+  Dual<N> r = dconst<N>(g.v);
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = (f0 + k < PF_COUNT) ? g.d[f0 + k] : 0.0;
+  return r;
 }
 
 // Density target of cell c: mean over the Sobol points of rho(R, z, phi) R,

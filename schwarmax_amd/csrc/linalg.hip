@@ -1090,9 +1090,9 @@ __global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
 // ---------------------------------------------------------------------------
 constexpr int INV_BASE = 128;
 
-// One workgroup of 256 threads, symmetric sweep operator on the (nb <= 128)
+// One workgroup of 1024 threads, symmetric sweep operator on the (nb <= 128)
 // block, read from the lower triangle and padded with the identity.  Thread t
-// keeps the 8 x 8 block (rows 8 (t / 16) + r, columns 8 (t % 16) + c) in
+// keeps the 4 x 4 block (rows 4 (t / 32) + r, columns 4 (t % 32) + c) in
 // registers.  Sweeping pivot k keeps the matrix symmetric, so pivot row and
 // pivot column are the same numbers: the threads holding column k publish it
 // to LDS, and every element updates from it:
@@ -1100,62 +1100,65 @@ constexpr int INV_BASE = 128;
 //   a_ij <- a_ij - a_ik a_kj / a_kk.
 // Sweeping all pivots leaves -A^-1; the full block is written back negated.
 // Every element takes the rank-1 update, then the threads holding the pivot
-// row or column overwrite those (so no wave runs two update paths).  The
-// published column alternates between two LDS buffers: a thread writing
-// pivot k's column has passed barrier k-1, which every reader of pivot k-2
-// (the same buffer) reached only after its reads, so one barrier per pivot
-// suffices.  Pivots run in groups of 8 with the position inside the group
-// unrolled, so every register index is static.
-__global__ __launch_bounds__(256) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
-  __shared__ double col[2][INV_BASE];
+// row or column overwrite those.  The published column alternates between
+// two LDS buffers: a thread writing pivot k's column has passed barrier k-1,
+// which every reader of pivot k-2 (the same buffer) reached only after its
+// reads, so one barrier per pivot suffices.  The sweep is bound by
+// instruction issue (64 FP64 FMAs per SIMD and pivot plus the per-wave
+// reads, division and patches): four waves per SIMD overlap one another's
+// latencies, where one wave of 8 x 8 blocks per SIMD waited on each.
+// Pivots run in groups of 4 with the position inside the group unrolled, so
+// every register index is static.
+__global__ __launch_bounds__(1024) void inv_base_kernel(double* __restrict__ A, int64_t lda, int nb) {
+  __shared__ __align__(16) double col[2][INV_BASE];
   const int t = threadIdx.x;
-  const int bi = t >> 4, bj = t & 15;
-  double a[8][8];
+  const int bi = t >> 5, bj = t & 31;
+  double a[4][4];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int i = 8 * bi + r, j = 8 * bj + c;
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * bi + r, j = 4 * bj + c;
       double x = (i == j) ? 1.0 : 0.0;
       if (i < nb && j < nb) x = (i >= j) ? A[(int64_t)i * lda + j] : A[(int64_t)j * lda + i];
       a[r][c] = x;
     }
-  for (int kb = 0; kb < INV_BASE / 8; ++kb) {
+  for (int kb = 0; kb < INV_BASE / 4; ++kb) {
 #pragma unroll
-    for (int kc = 0; kc < 8; ++kc) {
-      const int k = 8 * kb + kc;
+    for (int kc = 0; kc < 4; ++kc) {
+      const int k = 4 * kb + kc;
       double* cb = col[kc & 1];
       if (bj == kb) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) cb[8 * bi + r] = a[r][kc];
+        for (int r = 0; r < 4; ++r) cb[4 * bi + r] = a[r][kc];
       }
       __syncthreads();
       const double f = 1.0 / cb[k];
-      double gr[8], gc[8];
+      double gr[4], gc[4];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) gr[r] = cb[8 * bi + r] * f;
+      for (int r = 0; r < 4; ++r) gr[r] = cb[4 * bi + r] * f;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) gc[c] = cb[8 * bj + c];
+      for (int c = 0; c < 4; ++c) gc[c] = cb[4 * bj + c];
 #pragma unroll
-      for (int r = 0; r < 8; ++r)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) a[r][c] -= gr[r] * gc[c];
+        for (int c = 0; c < 4; ++c) a[r][c] -= gr[r] * gc[c];
       if (bi == kb) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) a[kc][c] = gc[c] * f;
+        for (int c = 0; c < 4; ++c) a[kc][c] = gc[c] * f;
       }
       if (bj == kb) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) a[r][kc] = gr[r];
+        for (int r = 0; r < 4; ++r) a[r][kc] = gr[r];
         if (bi == kb) a[kc][kc] = -f;
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int i = 8 * bi + r, j = 8 * bj + c;
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * bi + r, j = 4 * bj + c;
       if (i < nb && j < nb) A[(int64_t)i * lda + j] = -a[r][c];
     }
 }
@@ -1267,7 +1270,7 @@ static int blas_dsyrkx(hipStream_t st, bool trans, int64_t n, int64_t k, double 
 #endif
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
-    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(256), 0, st, A, lda, (int)n);
+    hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(1024), 0, st, A, lda, (int)n);
     SMX_LAUNCHED();
     return 0;
   }

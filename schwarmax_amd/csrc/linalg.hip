@@ -284,6 +284,16 @@ constexpr int AG_BN = 256;   // columns of M per workgroup
 constexpr int AG_OCC = 2;    // workgroups per CU (LDS and registers)
 constexpr int AG_CB = AG_BN / 128;   // 32-column blocks per wave
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
+// Waves per ADMM product workgroup: 8 waves of 128 x 32 outputs (128
+// VGPRs, four waves per SIMD at two workgroups per CU) run the product 4%
+// faster than 4 waves of 128 x 64 (240 VGPRs, two waves per SIMD): with M's
+// loads taken out the kernel still takes 80% of its time, so the chunk
+// pipeline (staging stores, barriers, fragment reads, MFMAs), not HBM, sets
+// it, and more waves hide more of its latencies.
+constexpr int AG_WAVES = 8;
+constexpr int AG_NT = 64 * AG_WAVES;            // its threads
+constexpr int AG_WCB = AG_BN / (32 * AG_WAVES); // 32-column blocks per wave
+static_assert(AG_WCB >= 1 && AG_WCB * 32 * AG_WAVES == AG_BN, "waves tile the 256 columns");
 constexpr float AG_TAU = 1.0f / 1024.0f;   // delta-product bound max|R - R_b| / max|R| per row
 
 // Full product this iteration?  force > 0: yes; force < 0: no (timing of the
@@ -334,8 +344,9 @@ __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t 
 
 // One chunk (KC reduction indices) of the ADMM product's operands.  B: 256
 // rows of M (hi and lo), thread t moves 16-byte segment t%TPR of row
-// t/TPR + (256/TPR)p, TPR = KC/8 lanes covering one row piece; A: 128 rows of
-// R, same pattern, split into hi and lo on the way to LDS.  The hi plane of
+// t/TPR + RPP p, TPR = KC/8 lanes covering one row piece, RPP = AG_NT/TPR
+// rows per pass; A: 128 rows of R, same pattern, split into hi and lo on the
+// way to LDS.  The hi plane of
 // the next chunk is loaded into registers before the MFMAs of the current
 // one; the lo plane and A (from L2) after them.  Prefetching both planes
 // does not fit the registers of two workgroups per CU.
@@ -343,14 +354,12 @@ template <int KC>
 struct AdmmStage {
   static_assert(KC == AG_KP, "the blocked split layouts use 32-wide chunks");
   static constexpr int TPR = KC / 8;
-  static constexpr int RPP = 256 / TPR;
+  static constexpr int RPP = AG_NT / TPR;
   static constexpr int PB_ = AG_BN / RPP;   // passes for B
   static constexpr int PA_ = AG_KR / RPP;   // passes for A
   uint4 bh[PB_], bl[PB_];
   // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
-  // row t/4 + 64p, segment t%4, i.e. the tile's uint4 number t + 256p
-  // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
-  // row t/4 + 64p, segment t%4, i.e. the tile's uint4 number t + 256p.
+  // row t/4 + RPP p, segment t%4, i.e. the tile's uint4 number t + AG_NT p.
   // M streams once per launch: non-temporal loads keep it from evicting the
   // R tiles that the workgroups of one split share in L2.
   __device__ static void load_plane(const _Float16* __restrict__ M, int64_t npad, int64_t j0,
@@ -421,7 +430,7 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
                                                const _Float16* __restrict__ Mh,
                                                const _Float16* __restrict__ Ml, AdmmStage<KC>& stg,
                                                _Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P],
-                                               _Float16 (*Al)[P], f32x16 (&acc)[4][AG_CB]) {
+                                               _Float16 (*Al)[P], f32x16 (&acc)[4][AG_WCB]) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 5, c = lane & 31;
   if (ch0 < ch1) {
@@ -439,12 +448,12 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
     }
 #pragma unroll
     for (int t = 0; t < KC / 16; ++t) {
-      half8 bh[AG_CB], bl[AG_CB];
+      half8 bh[AG_WCB], bl[AG_WCB];
 #pragma unroll
-      for (int cb = 0; cb < AG_CB; ++cb) {
-        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
+      for (int cb = 0; cb < AG_WCB; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / AG_WAVES) + cb * 32 + c][16 * t + 8 * g]);
         if (FULL)
-          bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / 4) + cb * 32 + c][16 * t + 8 * g]);
+          bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / AG_WAVES) + cb * 32 + c][16 * t + 8 * g]);
       }
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
@@ -452,7 +461,7 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t
         half8 al;
         if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
-        for (int cb = 0; cb < AG_CB; ++cb) {
+        for (int cb = 0; cb < AG_WCB; ++cb) {
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
           if (FULL) {
             acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
@@ -485,7 +494,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
 // D (scales from dmax), as admm_full_product decides.  D == nullptr: always
 // full (force must then be 1).
 template <int KC>
-__global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
+__global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
     int64_t K, int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
     const unsigned* __restrict__ rmax, const float* __restrict__ D,
     const unsigned* __restrict__ dmax, int force, const _Float16* __restrict__ Mh,
@@ -512,11 +521,11 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
   const int64_t nch = npad / KC;
   const int64_t ch0 = (int64_t)split * chunks_per_split;
   const int64_t ch1 = ch0 + chunks_per_split < nch ? ch0 + chunks_per_split : nch;
-  f32x16 acc[4][AG_CB];
+  f32x16 acc[4][AG_WCB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < AG_CB; ++b)
+    for (int b = 0; b < AG_WCB; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
   AdmmStage<KC> stg;
@@ -530,12 +539,12 @@ __global__ __launch_bounds__(256, AG_OCC) void admm_gemm_kernel(
   // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4,
   // column c.  Rows K..Kp-1 (zero padding of R) are not stored: every reader
   // of the partials sums rows k < K only.
-  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / 4);
+  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / AG_WAVES);
   const int64_t rows = K - k0;
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < AG_CB; ++cb)
+    for (int cb = 0; cb < AG_WCB; ++cb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
@@ -855,7 +864,7 @@ static void launch_admm_product(hipStream_t st, int64_t K, int64_t Kp, int64_t n
   const int64_t nch = npad / AG_KP;
   const int cps = static_cast<int>((nch + S - 1) / S);
   const dim3 grid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
-  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), grid, dim3(256), 0, st, K, Kp, npad, cps, R, rmax,
+  hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), grid, dim3(AG_NT), 0, st, K, Kp, npad, cps, R, rmax,
                      D, dmax, force, Mh, Ml, part);
 }
 

@@ -282,7 +282,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int AG_KR = 128;   // rows of R per workgroup
 constexpr int AG_BN = 256;   // columns of M per workgroup
 constexpr int AG_OCC = 2;    // workgroups per CU (LDS and registers)
-constexpr int AG_CB = AG_BN / 128;   // 32-column blocks per wave
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
 // Waves per ADMM product workgroup: 8 waves of 128 x 32 outputs (128
 // VGPRs, four waves per SIMD at two workgroups per CU) run the product 4%
@@ -932,6 +931,15 @@ __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
 // chunk c) = rows 128 I + r, indices 32 c + q at ((I nchk + c) 128 + r) 32 + q.
 // ---------------------------------------------------------------------------
 constexpr int GR_TM = 128;   // rows (A) per tile; the B operand is two such blocks
+// Waves per Gram workgroup.  Unlike the ADMM product, the Gram does three
+// MFMAs per fragment pair and every wave reads all 128 A rows from LDS: 8
+// waves of 128 x 32 outputs took 28.4 ms at plain-20k against 18.1 ms for 4
+// waves of 128 x 64.
+constexpr int GR_WAVES = 4;
+constexpr int GR_NT = 64 * GR_WAVES;
+constexpr int GR_WCB = 2 * GR_TM / (32 * GR_WAVES);     // 32-column blocks per wave
+constexpr int GR_PASSES = GR_TM / (GR_NT / 4);          // staging passes per 128-row tile
+static_assert(GR_WCB >= 1 && GR_PASSES >= 1, "Gram wave layout");
 
 __device__ __forceinline__ int64_t g_tile_index(int64_t i, int64_t k, int64_t nchk) {
   return ((i / GR_TM * nchk + k / AG_KP) * GR_TM + i % GR_TM) * AG_KP + k % AG_KP;
@@ -1002,7 +1010,7 @@ static int64_t gram_grid(int64_t gpad) { return 8 * ((gram_tiles(gpad) + 7) / 8)
 // One (128 x 256) tile of Q per workgroup (4 waves of 128 x 64), 32-index
 // chunks staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep step; writes
 // Q[i][j] for j <= i < n.
-__global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
+__global__ __launch_bounds__(GR_NT, AG_OCC) void gram_split_kernel(
     int64_t n, int64_t gpad, int64_t mpad, const _Float16* __restrict__ Uh,
     const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q) {
   constexpr int P = AG_KP + 8;
@@ -1019,46 +1027,45 @@ __global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
   const int g = lane >> 5, c = lane & 31;
   const int64_t nchk = mpad / AG_KP;
   const int64_t tileA = I * nchk, tileB0 = 2 * J * nchk, tileB1 = (2 * J + 1) * nchk;
-  const int r0 = tid >> 2, seg = 8 * (tid & 3);   // rows r0, r0 + 64 of a 128-row tile
-  f32x16 acc[4][AG_CB];
+  const int r0 = tid >> 2, seg = 8 * (tid & 3);   // rows r0 + (GR_NT / 4) h of a 128-row tile
+  f32x16 acc[4][GR_WCB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < AG_CB; ++b)
+    for (int b = 0; b < GR_WCB; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
   for (int64_t ch = 0; ch < nchk; ++ch) {
-    // stage chunk ch: each plane tile is 512 16-byte runs, thread t moves t and t + 256
+    // stage chunk ch: each plane tile is 512 16-byte runs, thread t moves runs t + GR_NT h
     const int64_t oa = ((tileA + ch) * GR_TM) * AG_KP;
     const int64_t ob0 = ((tileB0 + ch) * GR_TM) * AG_KP;
     const int64_t ob1 = ((tileB1 + ch) * GR_TM) * AG_KP;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int64_t e = (int64_t)(r0 + 64 * h) * AG_KP + seg;
-      *reinterpret_cast<uint4*>(&Ah[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Uh + oa + e);
-      *reinterpret_cast<uint4*>(&Al[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Ul + oa + e);
-      *reinterpret_cast<uint4*>(&Bh[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Uh + ob0 + e);
-      *reinterpret_cast<uint4*>(&Bl[r0 + 64 * h][seg]) = *reinterpret_cast<const uint4*>(Ul + ob0 + e);
-      *reinterpret_cast<uint4*>(&Bh[GR_TM + r0 + 64 * h][seg]) =
-          *reinterpret_cast<const uint4*>(Uh + ob1 + e);
-      *reinterpret_cast<uint4*>(&Bl[GR_TM + r0 + 64 * h][seg]) =
-          *reinterpret_cast<const uint4*>(Ul + ob1 + e);
+    for (int h = 0; h < GR_PASSES; ++h) {
+      const int rr = r0 + (GR_NT / 4) * h;
+      const int64_t e = (int64_t)rr * AG_KP + seg;
+      *reinterpret_cast<uint4*>(&Ah[rr][seg]) = *reinterpret_cast<const uint4*>(Uh + oa + e);
+      *reinterpret_cast<uint4*>(&Al[rr][seg]) = *reinterpret_cast<const uint4*>(Ul + oa + e);
+      *reinterpret_cast<uint4*>(&Bh[rr][seg]) = *reinterpret_cast<const uint4*>(Uh + ob0 + e);
+      *reinterpret_cast<uint4*>(&Bl[rr][seg]) = *reinterpret_cast<const uint4*>(Ul + ob0 + e);
+      *reinterpret_cast<uint4*>(&Bh[GR_TM + rr][seg]) = *reinterpret_cast<const uint4*>(Uh + ob1 + e);
+      *reinterpret_cast<uint4*>(&Bl[GR_TM + rr][seg]) = *reinterpret_cast<const uint4*>(Ul + ob1 + e);
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < AG_KP / 16; ++t) {
-      half8 bh[AG_CB], bl[AG_CB];
+      half8 bh[GR_WCB], bl[GR_WCB];
 #pragma unroll
-      for (int cb = 0; cb < AG_CB; ++cb) {
-        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
-        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * 64 + cb * 32 + c][16 * t + 8 * g]);
+      for (int cb = 0; cb < GR_WCB; ++cb) {
+        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (2 * GR_TM / GR_WAVES) + cb * 32 + c][16 * t + 8 * g]);
+        bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (2 * GR_TM / GR_WAVES) + cb * 32 + c][16 * t + 8 * g]);
       }
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
         const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
         const half8 al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
-        for (int cb = 0; cb < AG_CB; ++cb) {
+        for (int cb = 0; cb < GR_WCB; ++cb) {
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
@@ -1071,8 +1078,8 @@ __global__ __launch_bounds__(256, AG_OCC) void gram_split_kernel(
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < AG_CB; ++cb) {
-      const int64_t j = j0 + wave * 64 + cb * 32 + c;
+    for (int cb = 0; cb < GR_WCB; ++cb) {
+      const int64_t j = j0 + wave * (2 * GR_TM / GR_WAVES) + cb * 32 + c;
       const double sj = (j < n) ? static_cast<double>(us[j]) : 1.0;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -1349,7 +1356,7 @@ static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64
   hipLaunchKernelGGL(build_ut_split_kernel, dim3((unsigned)w.gpad), dim3(256), 0, st, lib, ld_lib,
                      n, m, ldu, w.mpad, row_scale, w.Ut, w.Uh, w.Ul, w.us);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(256), 0, st, n,
+  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(GR_NT), 0, st, n,
                      w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,

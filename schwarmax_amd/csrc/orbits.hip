@@ -127,40 +127,57 @@ __global__ __launch_bounds__(64, SMX_INTEGRATOR_WAVES) void integrate_library_ke
 // lib[c, :] = sum_r valid_r * (slab_r / T_r) / (sum_r valid_r + 0.1)
 // (integrates.py:212-236, 287-300); output row order
 // [Rzphi | surface density | h1 | h2 | h3 | h4 | h0].
+// One work item per (column, cell) and per (column, aperture): an aperture
+// item reads its 64-byte record once per realization and writes the six
+// aperture rows (counts, h1..h4, h0), rather than six items each reading one
+// value of the record (whose other values then come back from beyond L2 for
+// the other five).
 __global__ void finalize_library_kernel(const double* __restrict__ ws, int64_t n, int r, int P,
                                         int B, const double* __restrict__ valid,
                                         const double* __restrict__ T_int,
                                         const double* __restrict__ num_per_bin, double area_pixel,
                                         int pre_normalised, double* __restrict__ lib) {
   const int64_t rows = (int64_t)P + 6 * (int64_t)B;
+  const int64_t items = (int64_t)P + (int64_t)B;
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= n * rows) return;
-  const int64_t c = idx / rows;
-  const int64_t j = idx - c * rows;
+  if (idx >= n * items) return;
+  const int64_t c = idx / items;
+  const int64_t j = idx - c * items;
   const int64_t stride = slab_stride(P, B);
   double nv = 0.0;
   for (int k = 0; k < r; ++k) nv += valid[c * r + k];
   const double wt = 1.0 / (nv + 0.1);
-  double out = 0.0;
+  if (j < P) {
+    double out = 0.0;
+    for (int k = 0; k < r; ++k) {
+      const int64_t it = c * r + k;
+      const double T = pre_normalised ? 1.0 : T_int[it];
+      const double v = ws[it * stride + j] / T;
+      out += (valid[it] > 0.5 ? v : 0.0) * wt;
+    }
+    lib[c * rows + j] = out;
+    return;
+  }
+  const int64_t a = j - P;
+  const double area = num_per_bin[a] * area_pixel + kEps;
+  // output blocks: 0 counts (surface density), 1..4 -> h1..h4, 5 -> h0;
+  // record components: 0 counts, 1 h0, 2..5 h1..h4
+  double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < r; ++k) {
     const int64_t it = c * r + k;
-    const double* rz = ws + it * stride;
-    const double* ap = rz + slab_rz_len(P);
+    const double* rec = ws + it * stride + slab_rz_len(P) + 8 * a;
     const double T = pre_normalised ? 1.0 : T_int[it];
-    double v;
-    if (j < P) {
-      v = rz[j] / T;
-    } else {
-      const int64_t q = j - P;
-      const int64_t blk = q / B, a = q - blk * B;
-      // blocks: 0 counts (surface density), 1..4 -> h1..h4, 5 -> h0
-      const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
-      v = ap[8 * a + comp] / T;
-      if (blk == 0) v = v / (num_per_bin[a] * area_pixel + kEps);
+    const bool ok = valid[it] > 0.5;
+#pragma unroll
+    for (int blk = 0; blk < 6; ++blk) {
+      const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : blk + 1);
+      double v = rec[comp] / T;
+      if (blk == 0) v = v / area;
+      out[blk] += (ok ? v : 0.0) * wt;
     }
-    out += (valid[it] > 0.5 ? v : 0.0) * wt;
   }
-  lib[idx] = out;
+#pragma unroll
+  for (int blk = 0; blk < 6; ++blk) lib[c * rows + P + blk * (int64_t)B + a] = out[blk];
 }
 
 // Trajectory path (integrates.py:319-512): integrate storing every attempt's
@@ -390,10 +407,10 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
                      t_final, ckpt, ckpt_every);
   SMX_LAUNCHED();
   stage_end(ST_INTEGRATE, st, 1);
-  const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
+  const int64_t items = (int64_t)P + g->num_vbin;   // finalize work items per column
   stage_begin(ST_FINALIZE, st);
-  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * rows, 256)), dim3(256), 0, st,
-                     ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 0, lib);
+  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * items, 256)), dim3(256), 0, st, ws,
+                     n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 0, lib);
   SMX_LAUNCHED();
   stage_end(ST_FINALIZE, st, 1);
   return 0;
@@ -445,9 +462,9 @@ int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx
   hipLaunchKernelGGL(bin_traj_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *g, n_int,
                      s->N_max, o_traj, t_traj, T_int, bin_mapping, v0, sdisp, ws);
   SMX_LAUNCHED();
-  const int64_t rows = (int64_t)P + 6 * (int64_t)g->num_vbin;
-  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * rows, 256)), dim3(256), 0, st,
-                     ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 1, lib);
+  const int64_t items = (int64_t)P + g->num_vbin;   // finalize work items per column
+  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * items, 256)), dim3(256), 0, st, ws,
+                     n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 1, lib);
   SMX_LAUNCHED();
   return 0;
 }

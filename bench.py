@@ -303,8 +303,9 @@ def main():
                     help='only run a device copy of this many MiB (HBM counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the path shards by orbit columns, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: only the "
+              f"single-GPU path is built (it would shard by orbit columns, one model's "
+              f"integrations per GPU, with one all-reduce of the Gram)", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), 'needs a GPU'

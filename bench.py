@@ -304,8 +304,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: only the "
-              f"single-GPU path is built (it would shard by orbit columns, one model's "
-              f"integrations per GPU, with one all-reduce of the Gram)", file=sys.stderr)
+              f"single-GPU path is built (it would shard by orbit columns, each GPU "
+              f"integrating a share of the orbits, with one all-reduce of the Gram)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), 'needs a GPU'

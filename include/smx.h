@@ -98,6 +98,12 @@ int smx_initial_conditions(const smx_potential* p, const double* w0, const doubl
                            double* wb, double* T_total, double* dt0, void* stream);
 
 /* ---- orbit library (integrates.py:37-313) ------------------------------- */
+/* Every bin_mapping argument below holds nX*nY + 1 int32 entries: the pixel
+ * -> aperture map, then the entry read for samples off the sky field (the
+ * reference's sentinel -1, or for a map given without one the last pixel's
+ * aperture, which is where JAX's gather of index nX*nY clamps;
+ * integrates.py:132-143, utils.py:429).  Ids outside [0, num_vbin) are
+ * dropped, as segment_sum drops them. */
 /* Bytes of the per-integration accumulator workspace for n_int integrations. */
 int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int);
 /* Integrate n_int = n*r orbits with fused binning into the workspace, then

@@ -98,7 +98,7 @@ def rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max, dec_in=
         dt_st = dec_in['dt']
     dt_new = np.where(dt_st < 0, dt_min, np.where(dt_st > 0, dt_max, dtp))
     dt_out = np.where(accept, dt, 0.0)
-    dec = {'accept': accept, 'sel': sel, 'floor': floor, 'sf': sf_st, 'dt': dt_st}
+    dec = {'accept': accept, 'sel': sel, 'floor': floor, 'sf': sf_st, 'dt': dt_st, 'err': err_norm}
     return t_new, y_out, dt_new, k1_new, dt_out, dec
 
 
@@ -178,7 +178,10 @@ class Grids:
         vzr = (Rm[2, 0] * ys[3] + Rm[2, 1] * ys[4]) + Rm[2, 2] * ys[5]
         xy = assign_regular_grid(np.stack([xr, yr], axis=-1),
                                  self.XY_minmax[:, 0], self.XY_minmax[:, 1], self.nXY)
-        vbin = self.bin_mapping[xy]
+        # bin_mapping[XY_indices] (integrates.py:140): off-field samples carry
+        # id nX*nY; a map without the sentinel entry (utils.py:429 allows both
+        # lengths) is gathered with JAX's clamp of out-of-range indices.
+        vbin = self.bin_mapping[np.minimum(xy, self.bin_mapping.size - 1)]
         return rzphi, vbin, vzr * KPCGYR_TO_KMS
 
     def gh_terms(self, vbin, vlos):
@@ -230,9 +233,8 @@ def integrate_chunked(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, 
     T_acc = np.zeros(M)
     signs = SIGN_SYM if symmetry else SIGN_SYM[:1]
     nsym = signs.shape[0]
-    assert N_max % chunk_size == 0
     owner = np.broadcast_to(np.arange(M), (nsym, chunk_size, M)).reshape(-1)
-    for _ in range(N_max // chunk_size):
+    for _ in range(N_max // chunk_size):          # n_chunks = N_max // chunk_size (integrates.py:63)
         ys, dts = [], []
         for _ in range(chunk_size):
             t, y, dt, k1, dt_out, dec = rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol,
@@ -297,11 +299,13 @@ def realization_average(per, n, r):
 
 
 def integrate_withtraj(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol, rtol,
-                       dt_min, dt_max, grids):
+                       dt_min, dt_max, grids, record=None):
     """integrate_adaptive_barred_withtraj (integrates.py:319-512), M integrations.
 
     Returns per-integration binned quantities (no symmetry, dt pre-normalised)
     plus o_traj (M, N, 6) with velocities in km/s and t_traj (M, N).
+    ``record``: a dict that receives each attempt's error norm 'err' (M, N)
+    and accept flag 'accept' (M, N), for first-divergence reports.
     """
     w0 = np.asarray(w0, dtype=np.float64)
     M = w0.shape[0]
@@ -313,10 +317,18 @@ def integrate_withtraj(w0, acc_fn, pot_fn, N_max, T_total, dt_init, Omega, atol,
     dt = np.asarray(dt_init, dtype=np.float64) * np.ones(M)
     traj = np.empty((N_max, 6, M))
     dts = np.empty((N_max, M))
+    errs = np.empty((N_max, M))
+    accs = np.empty((N_max, M), dtype=bool)
     for i in range(N_max):
-        t, y, dt, k1, dt_out = rk_step(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min, dt_max)
+        t, y, dt, k1, dt_out, dec = rk_step_dec(t, y, dt, k1, acc_fn, Omega, atol, rtol, dt_min,
+                                                dt_max)
         traj[i] = y
         dts[i] = dt_out
+        errs[i] = dec['err']
+        accs[i] = dec['accept']
+    if record is not None:
+        record['err'] = errs.T.copy()
+        record['accept'] = accs.T.copy()
     t_traj = np.cumsum(dts, axis=0).T                     # (M, N)
     EJ = compute_EJ(y, pot_fn, Omega)
     valid = np.where((np.fabs(EJ / EJ0 - 1.0) < 0.1) & (t > np.asarray(T_total) / 10), 1.0, 0.0)

@@ -119,6 +119,26 @@ def _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_li
     return lib
 
 
+def library_guard(lib, min_valid=0.8):
+    """Check that an oracle library can make a parity comparison bite.
+
+    ``lib``: the library dict of build_model(..., return_library=True).  The
+    reference zeroes every integration that fails its validity gate
+    (|E_J/E_J0 - 1| < 0.1 and t_final > T_total / 10, integrates.py:209-236);
+    a library of such integrations is identically zero and a model built on
+    it does not depend on the integrator, the binning or the NNLS.  Raises
+    AssertionError unless at least ``min_valid`` of the integrations are valid
+    and the density, surface-density and h1..h4 rows all have non-zero entries.
+    Returns the valid fraction."""
+    frac = float(np.mean(lib['per']['valid']))
+    if frac < min_valid:
+        raise AssertionError(f"only {frac:.3f} of the integrations are valid (need {min_valid})")
+    for k in ('Rzphi', 'surface_density', 'h1', 'h2', 'h3', 'h4'):
+        if not np.any(np.asarray(lib[k]) != 0.0):
+            raise AssertionError(f"library rows {k} are all zero")
+    return frac
+
+
 def _targets(dict_data, params_disk_rho, n):
     """Density target and normalised data (model.py:192-258)."""
     lm = 1 / params_disk_rho['mass_to_light_ratio']

@@ -164,13 +164,41 @@ def check(rc, what):
         raise RuntimeError(f"{what} failed (code {rc}): {last_error()}")
 
 
-def ptr(t):
-    """Device pointer of a contiguous tensor (or None)."""
+def ptr(t, dtype=None):
+    """Device pointer of a contiguous tensor (or None).
+
+    ``dtype``: the element type the C entry point reads through this pointer
+    (torch.float64 for ``const double*``, torch.int32 for ``const int32_t*``,
+    ...); a tensor of another type raises TypeError instead of being read as
+    the wrong type or past its end."""
     if t is None:
         return None
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"tensor has dtype {t.dtype}, the entry point reads {dtype}; "
+                        f"convert it first (tensor.to({dtype}))")
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def bin_map(bin_mapping, nX, nY, device='cuda'):
+    """Pixel -> aperture map as the kernels read it: int32, nX*nY + 1 entries.
+
+    Entry nX*nY is read for samples off the sky field.  The reference indexes
+    bin_mapping with id nX*nY for such samples (integrates.py:132-143,
+    utils.py:40-42) and accepts maps of nX*nY or nX*nY + 1 entries (utils.py:429):
+    with the sentinel the entry there is used (normally -1, dropped); without it
+    JAX clamps the out-of-range gather to the last pixel's aperture, so that
+    aperture's id is appended here."""
+    bm = np.asarray(bin_mapping.cpu() if isinstance(bin_mapping, torch.Tensor) else bin_mapping)
+    bm = bm.astype(np.int32).reshape(-1)
+    npix = int(nX) * int(nY)
+    if bm.size < npix:
+        raise ValueError(f"bin_mapping has {bm.size} entries, the {nX} x {nY} field needs "
+                         f"{npix} (plus an optional sentinel)")
+    if bm.size == npix:
+        bm = np.concatenate([bm, bm[-1:]])
+    return torch.as_tensor(np.ascontiguousarray(bm[:npix + 1]), dtype=torch.int32, device=device)
 
 
 def stream_ptr(stream=None):

@@ -35,15 +35,19 @@ __device__ __forceinline__ int rzphi_cell(const smx_grids& g, double R, double z
   return a + b * g.nR + c * g.nR * g.nz;
 }
 
-// Aperture id of sky position (xr, yr), or -1 when off the field or mapped to
-// the sentinel (integrates.py:132-143).
+// Aperture id of sky position (xr, yr), or -1 when its map entry is not an
+// aperture (integrates.py:132-143).  An off-field sample takes the pixel id
+// nX*nY (utils.py:40-42), so it reads the map's last entry, bin_mapping[nX*nY]:
+// the sentinel -1 when the map carries one; the last pixel's aperture when the
+// host padded a map without one (the reference's gather clamps there).  The
+// map therefore always has nX*nY + 1 entries.
 __device__ __forceinline__ int aperture(const smx_grids& g, const int32_t* bin_mapping,
                                         double xr, double yr) {
   int a, b;
   const bool ok0 = grid_coord(xr, g.xy_min[0], g.xy_max[0], g.nX, a);
   const bool ok1 = grid_coord(yr, g.xy_min[1], g.xy_max[1], g.nY, b);
-  if (!(ok0 && ok1)) return -1;
-  const int v = bin_mapping[a + b * g.nX];
+  const int pix = (ok0 && ok1) ? a + b * g.nX : g.nX * g.nY;
+  const int v = bin_mapping[pix];
   return (v >= 0 && v < g.num_vbin) ? v : -1;
 }
 

@@ -154,12 +154,10 @@ class _Library(torch.autograd.Function):
         t_final = torch.empty(n_int, dtype=torch.float64, device='cuda')
         nck = (sched_struct.N_max + ckpt_every - 1) // ckpt_every
         ckpt = torch.empty((max(nck, 1), n_int, 8), dtype=torch.float64, device='cuda')
-        bm = _lib.dev(dict_data['bin_mapping'], dtype=torch.int32)
+        bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
         npb = _lib.dev(dict_data['num_per_bin'])
         v0 = _lib.dev(dict_data['v0'])
         sd = _lib.dev(dict_data['s'])
-        if bm.numel() < grids.nX * grids.nY:
-            raise ValueError("bin_mapping must have nX*nY entries (plus the sentinel)")
         wb = wb.contiguous()
         dt0 = dt0.contiguous()
         _lib.check(L.smx_build_library_ckpt(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),

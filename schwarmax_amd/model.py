@@ -71,9 +71,16 @@ def make_grids(num_Vbin, Rzphi_n_grid, Rzphi_lim_grid, xy_lim_grid, xy_n_grid, r
     return g
 
 
-def make_schedule(s, Omega):
+def make_schedule(s, Omega, chunked=True):
+    """The kernels' schedule.  The chunked integrator runs N_max // chunk
+    chunks of chunk attempts (integrates.py:63), so a N_max that is not a
+    multiple of the chunk size gives (N_max // chunk) * chunk attempts; the
+    trajectory integrator runs all N_max (integrates.py:382-384)."""
     sc = _lib.SmxSchedule()
-    sc.N_max = int(s['N_max'])
+    N = int(s['N_max'])
+    if chunked:
+        N = (N // int(s['chunk'])) * int(s['chunk'])
+    sc.N_max = N
     sc.Omega = float(Omega)
     sc.atol, sc.rtol = float(s['atol']), float(s['rtol'])
     sc.dt_min, sc.dt_max = float(s['dt_min']), float(s['dt_max'])
@@ -131,12 +138,10 @@ def build_library(pot, grids, sched_struct, wb, Tt, dt0, dict_data, traj=False):
     lib = torch.empty((n, P + 6 * B), dtype=torch.float64, device='cuda')
     valid = torch.empty(n_int, dtype=torch.float64, device='cuda')
     T_int = torch.empty(n_int, dtype=torch.float64, device='cuda')
-    bm = _lib.dev(dict_data['bin_mapping'], dtype=torch.int32)
+    bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
     npb = _lib.dev(dict_data['num_per_bin'])
     v0 = _lib.dev(dict_data['v0'])
     sd = _lib.dev(dict_data['s'])
-    if bm.numel() < grids.nX * grids.nY:
-        raise ValueError("bin_mapping must have nX*nY entries (plus the sentinel)")
     extra = {}
     if traj:
         N = sched_struct.N_max
@@ -206,7 +211,8 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     if grids.nR * grids.nz * grids.nphi != int(Rzphi_n_tot):
         raise ValueError("Rzphi_n_tot must equal prod(Rzphi_n_grid)")
     wb, Tt, dt0 = initial_conditions(pot, dict_data, r, sched)
-    libd = build_library(pot, grids, make_schedule(sched, -Omega_bar), wb, Tt, dt0, dict_data, traj)
+    libd = build_library(pot, grids, make_schedule(sched, -Omega_bar, chunked=not traj), wb, Tt,
+                         dt0, dict_data, traj)
     out, aux = model_from_library(libd['lib'], libd['P'], libd['B'], params_disk_rho, dict_data,
                                   nnls_maxiter, regularization, channel_w, nnls_tol)
     aux.update(libd=libd, rot=rot, Omega_bar=Omega_bar, wb=wb, T_total=Tt, dt0=dt0)

@@ -28,8 +28,15 @@ ARCSEC_RAD = math.pi / (180.0 * 3600.0)
 # schedule maps "P bar periods, S samples per orbit" onto the reference's
 # N_dynamical_time (validity horizon, in orbital times) and N_max (number of
 # BS3 attempts), keeping its dt_init = T_orb / 100.
+#
+# plain-2k (1,000 samples at rtol 1e-6) takes its horizon from the reference's
+# own rule N_max = N_step_per_orb * N_dynamical_time (model.py:163-165), so
+# N_dynamical_time = 10.  At rtol 1e-6 the BS3 controller takes about 385
+# attempts per orbital time, so 1,000 attempts cover a median 2.6 orbital times
+# and pass the validity gate t_final > T_total / 10 (integrates.py:210) in 99% of
+# integrations; a 50-period horizon would fail it in all of them.
 CONFIGS = {
-    'plain-2k': dict(n=2000, N_max=1000, N_dyn=50, rtol=1e-6, n_ap=300, field_arcsec=40,
+    'plain-2k': dict(n=2000, N_max=1000, N_dyn=10, rtol=1e-6, n_ap=300, field_arcsec=40,
                      grid=(20, 10, 16), K=1, T=500, seeded_params=False),
     'plain-20k': dict(n=20000, N_max=2000, N_dyn=100, rtol=1e-4, n_ap=1000, field_arcsec=60,
                       grid=(32, 16, 24), K=100, T=1000, seeded_params=True),
@@ -116,8 +123,12 @@ def disc_positions(n, rng, R_d=3.0, h_z=0.3, R_max=12.0):
 
 
 def make_galaxy(config='plain-2k', seed=0, n=None, n_ap=None, grid=None, K=None, nxy=None,
-                distance_mpc=50.0, r=4):
+                distance_mpc=50.0, r=4, N_max=None):
     """Geometry, positions and noise draws for one galaxy (observations empty).
+
+    ``N_max``: a shorter (or longer) schedule than the configuration's, with
+    the horizon set by the reference's rule N_dynamical_time = N_max / 100
+    (model.py:163-165).
 
     Returns (theta, dict_data, schedule).
     """
@@ -153,6 +164,9 @@ def make_galaxy(config='plain-2k', seed=0, n=None, n_ap=None, grid=None, K=None,
     sched = dict(N_max=cfg['N_max'], N_dynamical_time=cfg['N_dyn'], N_step_per_orb=100,
                  rtol=cfg['rtol'], atol=1e-7, dt_min=1e-5, dt_max=0.3, chunk=100,
                  n_realizations=r)
+    if N_max is not None:
+        sched['N_max'] = int(N_max)
+        sched['N_dynamical_time'] = int(N_max) / sched['N_step_per_orb']
     return theta, dd, sched
 
 

@@ -57,3 +57,58 @@ def test_noncanonical_potential_is_rejected():
     from schwarmax_amd import model as M
     with pytest.raises(NotImplementedError):
         M._canonical(lambda *a: 0, lambda *a: 0)
+
+
+def test_pointer_dtype_is_checked():
+    """A float32 / int32 buffer handed to a double* parameter raises instead of
+    being read as doubles (the reference's JAX buffers are float32 without x64)."""
+    import pytest
+    import torch
+    from schwarmax_amd import _lib
+    f32 = torch.zeros(8, dtype=torch.float32)
+    with pytest.raises(TypeError, match="float64"):
+        _lib.ptr(f32, torch.float64)
+    with pytest.raises(TypeError, match="int32"):
+        _lib.ptr(torch.zeros(4, dtype=torch.int64), torch.int32)
+    assert _lib.ptr(f32.to(torch.float64), torch.float64).value is not None
+    assert _lib.ptr(None, torch.float64) is None
+    # the Python drop-ins convert user arrays of any float type to float64
+    got = _lib.dev(np.arange(5, dtype=np.float32), device='cpu')
+    assert got.dtype == torch.float64 and np.array_equal(got.numpy(), np.arange(5.0))
+
+
+def test_bin_map_with_and_without_sentinel():
+    """Off-field samples read entry nX*nY: the sentinel when present; without
+    it, JAX's clamped gather of index nX*nY, i.e. the last pixel's aperture."""
+    import pytest
+    from schwarmax_amd import _lib
+    from oracle import integrate as OI
+    nX, nY = 4, 3
+    bm = np.arange(nX * nY, dtype=np.int64) % 5
+    padded = _lib.bin_map(bm, nX, nY, device='cpu').numpy()
+    assert padded.dtype == np.int32 and padded.size == nX * nY + 1
+    assert np.array_equal(padded[:-1], bm) and padded[-1] == bm[-1]
+    with_sent = np.concatenate([bm, [-1]])
+    assert np.array_equal(_lib.bin_map(with_sent, nX, nY, device='cpu').numpy(), with_sent)
+    with pytest.raises(ValueError):
+        _lib.bin_map(bm[:-1], nX, nY, device='cpu')
+    # the oracle's gather: an off-field sample with and without the sentinel
+    args = (5, None, np.ones(5), [[0, 10], [-3, 3], [-np.pi, np.pi]], [[-1, 1], [-1, 1]],
+            [2, 2, 2], [nX, nY], np.zeros(5), np.ones(5), np.eye(3))
+    ys = np.zeros((6, 2))
+    ys[0] = [0.1, 5.0]                      # on the field, off the field
+    for m, off in ((with_sent, -1), (bm, bm[-1])):
+        g = OI.Grids(args[0], m, *args[2:])
+        _, vb, _ = g.indices(ys)
+        assert vb[1] == off, (m.size, vb)
+        assert vb[0] == m[2 + 1 * nX]         # x=0.1 -> ix=2, y=0 -> iy=1
+
+
+def test_chunked_schedule_rounds_to_whole_chunks():
+    """integrates.py:63 runs N_max // chunk_size chunks; the trajectory path
+    runs all N_max attempts."""
+    from schwarmax_amd import model as M
+    s = M.schedule_of(dict(N_max=250, chunk=100))
+    assert M.make_schedule(s, 1.0).N_max == 200
+    assert M.make_schedule(s, 1.0, chunked=False).N_max == 250
+    assert M.make_schedule(M.schedule_of(None), 1.0).N_max == 5000

@@ -1,8 +1,19 @@
-"""Batched evaluation (GPU): smx_build_library_batch / model_likelihood_batch
-against per-model model_likelihood on the same orbits (uneven orbit counts and
-step counts, different theta)."""
+"""Batched evaluation (GPU): model_likelihood_batch / smx_build_library_batch
+against the FP64 oracle, model by model, with different theta, uneven orbit
+counts and uneven step counts (N_max 500 and 300).  Every model's oracle
+library passes ``library_guard`` (>= 80% valid integrations, non-zero rows)
+before it is compared.
+
+Bars: logL and the NNLS weights within 1e-4 relative of the oracle's; library
+columns (realization averages of four integrations each) within 1e-8 for at
+least 90% of them (an accept decision flipping at err_norm = 1 diverges a
+column, see test_gpu_model).  The batched values also equal the GPU's
+per-model path to 1e-12: the same kernels in another launch shape.
+"""
 import numpy as np
 import pytest
+
+from galaxies import observed_galaxy, oracle_likelihood, library_rows, column_errors, rowwise_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -14,27 +25,12 @@ def setup():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    from schwarmax_amd import synth, likelihood as L
-    from oracle import model as OM, potential as OP
-    theta, dd, sched = synth.make_galaxy('plain-2k', seed=11, n=40, n_ap=NAP, grid=(8, 4, 6), K=5,
-                                         nxy=16)
-    sched['N_max'] = 500
-    ph, pb = L.theta_to_params(theta)
-    Rl, Rn, xyl, xyn = L.grid_args(dd)
-    blank = {'XY_density_data': np.ones(NAP), 'XY_density_data_err': np.full(NAP, 0.01)}
-    for k in (1, 2, 3, 4):
-        blank['h%d_data' % k] = np.zeros(NAP)
-        blank['h%d_data_err' % k] = np.full(NAP, 0.03)
-    _, lib = OM.build_model(OP.density_func, OP.potential_func, ph, pb, {**dd, **blank}, NAP,
-                            dd['Rzphi_n_tot'], Rn, Rl, xyl, xyn, nnls_maxiter=2, schedule=sched,
-                            return_library=True)
-    A_xy = lib['surface_density'].T
-    A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
-    y_R = OM.density_target(dd, pb)
-    synth.observe(dd, A_xy, A_h, y_R.sum() / A_xy.shape[1], pb['mass_to_light_ratio'], floor=1e-3)
+    from schwarmax_amd import synth
+    theta, dd, sched, _, _ = observed_galaxy(seed=11, n=40, n_ap=NAP, K=5, N_max=500)
     rng = np.random.default_rng(8)
     thetas = np.stack([theta] + synth.batched_thetas(2, rng))
-    return thetas, dd, sched
+    s2 = dict(sched, N_max=300, N_dynamical_time=3)           # uneven step counts
+    return thetas, dd, [sched, sched, s2]
 
 
 def _first(dd, n):
@@ -44,38 +40,41 @@ def _first(dd, n):
     return out
 
 
-def test_batch_matches_per_model(setup):
-    from schwarmax_amd import batch, likelihood as L, model as M
-    thetas, dd, sched = setup
+def test_batch_matches_oracle(setup):
+    from schwarmax_amd import batch, likelihood as L
+    thetas, dd, scheds = setup
     n_orbits = [40, 25, 33]
-    s2 = dict(sched)
-    s2['N_max'] = 300                    # uneven step counts
-    scheds = [sched, sched, s2]
     got, aux = batch.model_likelihood_batch(thetas, dd, NAP, dd['Rzphi_n_tot'], n_orbits=n_orbits,
-                                            schedule=scheds, nnls_maxiter=100, return_aux=True)
+                                            schedule=scheds, nnls_maxiter=150, return_aux=True)
     got = got.cpu().numpy()
-    Rl, Rn, xyl, xyn = L.grid_args(dd)
     for m in range(3):
         dm = _first(dd, n_orbits[m])
-        ref = float(L.model_likelihood(thetas[m], dm, NAP, dd['Rzphi_n_tot'], schedule=scheds[m],
-                                       nnls_maxiter=100))
-        assert abs(got[m] - ref) <= 1e-12 * abs(ref), (m, got[m], ref)
-        ph, pb = L.theta_to_params(thetas[m])
-        _, a = M.build_model(L.density_func, L.potential_func, ph, pb, dm, NAP, dd['Rzphi_n_tot'],
-                             Rn, Rl, xyl, xyn, nnls_maxiter=2, schedule=scheds[m], return_aux=True)
-        lib_ref = a['libd']['lib'].cpu().numpy()
+        ref, md, lib, frac = oracle_likelihood(thetas[m], dm, NAP, scheds[m], 150)
         lib_b = aux[m]['lib'].cpu().numpy()
-        assert lib_b.shape == lib_ref.shape
-        assert np.max(np.abs(lib_b - lib_ref)) <= 1e-13 * np.max(np.abs(lib_ref)), m
+        col = column_errors(lib_b, library_rows(lib))
+        w_err = rowwise_rel(aux[m]['weights'].cpu().numpy(), md['weights'])
+        print(f"model {m}: n={n_orbits[m]} N_max={scheds[m]['N_max']} valid={frac:.3f} "
+              f"logL gpu={got[m]:.10g} oracle={ref:.10g} rel={abs(got[m] - ref) / abs(ref):.2e} "
+              f"weights {w_err:.2e} library columns: 90% {np.quantile(col, 0.9):.2e} "
+              f"max {col.max():.2e}", flush=True)
+        assert np.quantile(col, 0.9) < 1e-8, np.sort(col)[-5:]
+        if col.max() < 1e-8:                   # no decision flipped: the solve sees the same data
+            assert w_err < 1e-4, (m, w_err)
+        assert abs(got[m] - ref) <= 1e-4 * abs(ref), (m, got[m], ref)
+        # the single-model GPU path: same kernels, same values
+        single = float(L.model_likelihood(thetas[m], dm, NAP, dd['Rzphi_n_tot'], schedule=scheds[m],
+                                          nnls_maxiter=150))
+        assert abs(got[m] - single) <= 1e-12 * abs(single), (m, got[m], single)
 
 
 def test_batch_groups_and_budget(setup):
     """A tiny library budget forces one model per group: same values."""
     from schwarmax_amd import batch
-    thetas, dd, sched = setup
+    thetas, dd, scheds = setup
     a = batch.model_likelihood_batch(thetas, dd, NAP, dd['Rzphi_n_tot'], n_orbits=[40, 25, 33],
-                                     schedule=sched, nnls_maxiter=50).cpu().numpy()
+                                     schedule=scheds, nnls_maxiter=50).cpu().numpy()
     b = batch.model_likelihood_batch(thetas, dd, NAP, dd['Rzphi_n_tot'], n_orbits=[40, 25, 33],
-                                     schedule=sched, nnls_maxiter=50,
+                                     schedule=scheds, nnls_maxiter=50,
                                      library_budget_bytes=1).cpu().numpy()
+    assert np.all(np.isfinite(a))
     assert np.array_equal(a, b)

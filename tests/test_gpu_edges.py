@@ -36,7 +36,9 @@ def test_single_rhs_solver(cuda):
                              lambda_reg=1.0, maxiter=300, w_rzphi=w5, w_xy=w5,
                              w_h=1.0).cpu().numpy()
     assert got.shape == ref.shape == (257,)
-    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-3
+    e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("weights: normwise relative error %.2e" % e)
+    assert e < 1e-4, e
     U = ON.design_matrix(A_R, A_xy, A_h, sR, sxy, sh, gamma, w5, w5, 1.0)
     Y = ON.rhs_vectors(y_R, sR, y_xy[None], [y[None] for y in y_h], sxy, sh, w5, w5, 1.0)
     assert _fit_err(U, Y, got[None], ref[None]) < 1e-4
@@ -70,7 +72,8 @@ def test_ragged_shapes(cuda, n, K):
                                        w_h=1.0).cpu().numpy()
     assert got.shape == (K, n)
     err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    assert err.max() < 1e-3, err
+    print("weights: normwise relative error per right-hand side, max %.2e" % err.max())
+    assert err.max() < 1e-4, err
     U = ON.design_matrix(A_R, A_xy, A_h, sR, sxy, sh, gamma, w5, w5, 1.0)
     Y = ON.rhs_vectors(y_R, sR, yb_xy, yb_h, sxy, sh, w5, w5, 1.0)
     assert _fit_err(U, Y, got, ref) < 1e-4
@@ -96,4 +99,5 @@ def test_rhs_without_kinematic_data(cuda):
                                        w_h=1.0).cpu().numpy()
     assert np.all(np.isfinite(got)) and np.all(got >= 0.0)
     err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    assert err.max() < 1e-3, err
+    print("weights: normwise relative error per right-hand side, max %.2e" % err.max())
+    assert err.max() < 1e-4, err

@@ -16,25 +16,9 @@ def galaxy():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    from schwarmax_amd import synth, likelihood as L
-    from oracle import model as OM, potential as OP
-    theta, dd, sched = synth.make_galaxy('plain-2k', seed=4, n=16, n_ap=NAP, grid=(5, 4, 4), K=2,
-                                         nxy=10)
-    sched['N_max'] = 200
-    sched['N_dynamical_time'] = 5
-    ph, pb = L.theta_to_params(theta)
-    Rl, Rn, xyl, xyn = L.grid_args(dd)
-    blank = {'XY_density_data': np.ones(NAP), 'XY_density_data_err': np.full(NAP, 0.01)}
-    for k in (1, 2, 3, 4):
-        blank['h%d_data' % k] = np.zeros(NAP)
-        blank['h%d_data_err' % k] = np.full(NAP, 0.03)
-    _, lib = OM.build_model(OP.density_func, OP.potential_func, ph, pb, {**dd, **blank}, NAP,
-                            dd['Rzphi_n_tot'], Rn, Rl, xyl, xyn, nnls_maxiter=2, schedule=sched,
-                            return_library=True)
-    A_xy = lib['surface_density'].T
-    A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
-    y_R = OM.density_target(dd, pb)
-    synth.observe(dd, A_xy, A_h, y_R.sum() / A_xy.shape[1], pb['mass_to_light_ratio'], floor=1e-3)
+    from galaxies import observed_galaxy
+    theta, dd, sched, _, _ = observed_galaxy(seed=4, n=16, n_ap=NAP, grid=(5, 4, 4), K=2, nxy=10,
+                                             N_max=200)
     return theta, dd, sched
 
 

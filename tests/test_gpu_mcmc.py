@@ -13,24 +13,9 @@ def galaxy():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    from schwarmax_amd import synth, likelihood as L
-    from oracle import model as OM, potential as OP
-    theta, dd, sched = synth.make_galaxy('plain-2k', seed=21, n=24, n_ap=NAP, grid=(6, 4, 4), K=3,
-                                         nxy=12)
-    sched['N_max'] = 200
-    sched['N_dynamical_time'] = 5
-    ph, pb = L.theta_to_params(theta)
-    Rl, Rn, xyl, xyn = L.grid_args(dd)
-    blank = {'XY_density_data': np.ones(NAP), 'XY_density_data_err': np.full(NAP, 0.01)}
-    for k in (1, 2, 3, 4):
-        blank['h%d_data' % k] = np.zeros(NAP)
-        blank['h%d_data_err' % k] = np.full(NAP, 0.03)
-    _, lib = OM.build_model(OP.density_func, OP.potential_func, ph, pb, {**dd, **blank}, NAP,
-                            dd['Rzphi_n_tot'], Rn, Rl, xyl, xyn, nnls_maxiter=2, schedule=sched,
-                            return_library=True)
-    synth.observe(dd, lib['surface_density'].T, [lib['h%d' % k].T for k in (1, 2, 3, 4)],
-                  OM.density_target(dd, pb).sum() / lib['surface_density'].shape[0],
-                  pb['mass_to_light_ratio'], floor=1e-3)
+    from galaxies import observed_galaxy
+    theta, dd, sched, _, _ = observed_galaxy(seed=21, n=24, n_ap=NAP, grid=(6, 4, 4), K=3, nxy=12,
+                                             N_max=200)
     return theta, dd, sched
 
 
@@ -54,3 +39,11 @@ def test_rmh_on_batched_likelihood(galaxy):
         # the batched ICs come from their own kernel; last-bit differences grow
         # along the orbits to ~1e-12 of logL
         assert abs(got - ref) <= 1e-10 * abs(ref), (c, got, ref)
+    # and against the oracle: the last position of chain 0 (a proposal the
+    # batched likelihood evaluated, away from theta)
+    from galaxies import oracle_likelihood
+    th = out['chain'][-1, 0]
+    ref_o = oracle_likelihood(th, dd, NAP, sched, 50)[0] - log_vol
+    got = out['logprob'][-1, 0]
+    print("chain 0 vs oracle", got, ref_o, flush=True)
+    assert abs(got - ref_o) <= 1e-4 * abs(ref_o), (got, ref_o)

@@ -40,7 +40,8 @@ def test_bootstrap_weights(maxiter):
                                        w_h=1.0).cpu().numpy()
     # FP32 factor / products against the FP64 oracle: normwise per right-hand side
     err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    assert err.max() < 1e-3, err
+    print("weights: normwise relative error per right-hand side, max %.2e" % err.max())
+    assert err.max() < 1e-4, err
     # and the fit quality (the chi^2 the likelihood sees) to 1e-4
     U = ON.design_matrix(A_R, A_xy, A_h, sR, sxy, sh, gamma, w5, w5, 1.0)
     Y = ON.rhs_vectors(y_R, sR, yb_xy, yb_h, sxy, sh, w5, w5, 1.0)
@@ -77,7 +78,8 @@ def test_delta_products_exercised():
     ref = ON.solve_nnls_admm_bootstrap(A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy, sh,
                                        lambda_reg=1.0, maxiter=maxiter, w_rzphi=w5, w_xy=w5, w_h=1.0)
     err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    assert err.max() < 1e-3, err
+    print("weights: normwise relative error per right-hand side, max %.2e" % err.max())
+    assert err.max() < 1e-4, err
     U = ON.design_matrix(A_R, A_xy, A_h, sR, sxy, sh, gamma, w5, w5, 1.0)
     Y = ON.rhs_vectors(y_R, sR, yb_xy, yb_h, sxy, sh, w5, w5, 1.0)
     f_ref = np.sum((ref @ U.T - Y) ** 2, axis=1)

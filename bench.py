@@ -30,30 +30,12 @@ def log(*a):
 
 
 def make_workload(config, seed, n=None):
-    import torch
-    from schwarmax_amd import synth, model as M, likelihood as L, potentials as POT
-    theta, dd, sched = synth.make_galaxy(config, seed=seed, n=n)
-    ph, pb = L.theta_to_params(theta)
-    s = M.schedule_of(sched)
-    pot = POT.pack_potential(M.baryon_params(pb), ph)
+    from schwarmax_amd import synth
     t0 = time.time()
-    wb, Tt, dt0 = M.initial_conditions(pot, dd, s['n_realizations'], s)
-    rot = M.makeRotationMatrix(pb['alpha'], pb['beta'], pb['gamma'])
-    synth.fit_base_gaussians(dd, wb[:, 0, :].cpu().numpy(), rot)
-    Rl, Rn, xyl, xyn = L.grid_args(dd)
-    g = M.make_grids(int(dd['total_bins']), Rn, Rl, xyl, xyn, rot, True)
-    libd = M.build_library(pot, g, M.make_schedule(s, -pb['Omega_bar']), wb, Tt, dt0, dd)
-    lib = libd['lib'].cpu().numpy()
-    P, B = libd['P'], libd['B']
-    A_xy = lib[:, P:P + B].T
-    A_h = [lib[:, P + B * k:P + B * (k + 1)].T for k in (1, 2, 3, 4)]
-    y_R = M.density_target(pb, dd).cpu().numpy()
-    synth.observe(dd, A_xy, A_h, y_R.sum() / lib.shape[0], pb['mass_to_light_ratio'], floor=1e-3)
-    valid = float(libd['valid'].mean().item())
-    del libd, wb, Tt, dt0
-    torch.cuda.empty_cache()
-    log(f'workload {config}: n={dd["w0"].shape[0]} B={dd["total_bins"]} P={P} '
-        f'K={dd["XY_standard_normal"].shape[0]} valid={valid:.3f} setup {time.time() - t0:.1f}s')
+    theta, dd, sched, info = synth.observed_workload(config, seed, n=n)
+    log(f'workload {config}: n={dd["w0"].shape[0]} B={dd["total_bins"]} P={info["P"]} '
+        f'K={dd["XY_standard_normal"].shape[0]} valid={info["valid"]:.3f} '
+        f'setup {time.time() - t0:.1f}s')
     return theta, dd, sched
 
 

@@ -267,6 +267,14 @@ int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* wor
  * started afterwards (workspaces are sized for the largest setting). */
 int smx_admm_set_occupancy(int32_t workgroups_per_cu);
 int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
+/* Delta-product bound tau in [0, 0.25] (default 2^-10): an ADMM iteration
+ * multiplies only R - R_b by the inverse's fp16 hi plane while every row has
+ * max|R_k - R_b,k| <= tau max|R_k|, and takes the full split product (and a
+ * new base R_b) otherwise; 0 makes every product full.  The delta product's
+ * error is about 2^-11 |M| |R - R_b|, so tau trades accuracy for bytes.
+ * Process-wide, atomic; applies to solves started afterwards. */
+int smx_admm_set_delta_threshold(double tau);
+int smx_admm_get_delta_threshold(double* tau);
 /* Mean time of `reps` ADMM product launches (full = 1: the full split
  * product; 0: the delta product) on the buffers of a workspace that a
  * smx_nnls_admm call of the same (n, m, K) has filled, with the current

@@ -177,11 +177,25 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     sched = schedule_of(schedule)
     pb = baryon_params(params_disk_rho)
     ph = params_halo_pot
-    lm = 1 / params_disk_rho['mass_to_light_ratio']
     Omega_bar = params_disk_rho['Omega_bar']
     rot = make_rotation_matrix(params_disk_rho['alpha'], params_disk_rho['beta'], params_disk_rho['gamma'])
     lib = _library(dict_data, pb, ph, Omega_bar, rot, num_Vbin, Rzphi_n_grid, Rzphi_lim_grid,
                    xy_lim_grid, xy_n_grid, sched, traj, tape)
+    out, aux = model_from_library(lib, dict_data, params_disk_rho, nnls_maxiter, regularization,
+                                  channel_w, W_fixed=W_fixed)
+    aux.update(rot=rot, Omega_bar=Omega_bar)
+    return out, lib, aux
+
+
+def model_from_library(lib, dict_data, params_disk_rho, nnls_maxiter, regularization=1.0,
+                       channel_w=(np.sqrt(5.0), np.sqrt(5.0), 1.0), W_fixed=None, admm=None):
+    """Everything build_model does after the library (model.py:182-286).
+
+    ``lib``: dict of library columns (n, .) 'Rzphi', 'surface_density',
+    'h1'..'h4'.  ``admm``: None for the oracle's own ADMM; or a callable
+    admm(U, Y, lambda_reg, maxiter, w_init) -> W (K, n) that solves the same
+    iterate sequence (nnls.py:101-163) from the design matrix and right-hand
+    sides built here.  Returns (model dict, aux)."""
     A_Rzphi = lib['Rzphi'].T
     A_xy = lib['surface_density'].T
     A_h = [lib['h%d' % k].T for k in (1, 2, 3, 4)]
@@ -191,11 +205,17 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     y_Rzphi, sig_Rzphi, y_xy, sig_xy = t['y_Rzphi'], t['sig_Rzphi'], t['y_xy'], t['sig_xy']
     y_h, sig_h, mmpo, gamma_kin = t['y_h'], t['sig_h'], t['mmpo'], t['gamma_kin']
     v0_ap, s_ap = t['v0_ap'], t['s_ap']
+    lm = t['lm']
     dd = t['dd']
     y_xy_boot = y_xy[None, :] + dd['XY_standard_normal'] * sig_xy[None, :]
     y_h_boot = [y_h[k][None, :] + dd['h%d_standard_normal' % (k + 1)] * sig_h[k][None, :]
                 for k in range(4)]
-    if W_fixed is None:
+    if W_fixed is None and admm is not None:
+        U = NN.design_matrix(A_Rzphi, A_xy, A_h, sig_Rzphi, sig_xy, sig_h, gamma_kin, *channel_w)
+        Y = NN.rhs_vectors(y_Rzphi, sig_Rzphi, y_xy_boot, y_h_boot, sig_xy, sig_h, *channel_w)
+        W = np.asarray(admm(U, Y, regularization, nnls_maxiter, np.sum(y_Rzphi) / n),
+                       dtype=np.float64)
+    elif W_fixed is None:
         W = NN.solve_nnls_admm_bootstrap(A_Rzphi, A_xy, A_h, y_Rzphi, y_xy, gamma_kin,
                                          y_xy_boot, y_h_boot, sig_Rzphi, sig_xy, sig_h,
                                          lambda_reg=regularization, maxiter=nnls_maxiter,
@@ -206,8 +226,8 @@ def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dic
     d2 = d2 * mmpo * lm
     out = {'density_3d': y_Rzphi, 'density_3d_model': d3, 'surface_density': d2,
            'h1': h1, 'h2': h2, 'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
-    return out, lib, dict(mmpo=mmpo, rot=rot, Omega_bar=Omega_bar, n=n, gamma_kin=gamma_kin,
-                          sig_Rzphi=sig_Rzphi, sig_xy=sig_xy, sig_h=sig_h)
+    return out, dict(mmpo=mmpo, n=n, gamma_kin=gamma_kin, sig_Rzphi=sig_Rzphi, sig_xy=sig_xy,
+                     sig_h=sig_h)
 
 
 def build_model(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,

@@ -116,6 +116,8 @@ _SIGS = [
     ('smx_nnls_admm_full_products', ctypes.c_int, [_I64, _I64, _I64, _P, ctypes.POINTER(_I32), _P]),
     ('smx_admm_set_occupancy', ctypes.c_int, [_I32]),
     ('smx_admm_get_occupancy', ctypes.c_int, [ctypes.POINTER(_I32)]),
+    ('smx_admm_set_delta_threshold', ctypes.c_int, [ctypes.c_double]),
+    ('smx_admm_get_delta_threshold', ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     ('smx_admm_product_time', ctypes.c_int,
      [_I64, _I64, _I64, _P, _I32, _I32, ctypes.POINTER(ctypes.c_float), _P]),
     ('smx_gemm_nt_f32', ctypes.c_int, [_I64, _I64, _I64, ctypes.c_float, _P, _I64, _P, _I64,

@@ -10,11 +10,17 @@
 //                     (the reference factors Q + rho I once and solves with
 //                     the factor every iteration; the explicit inverse makes
 //                     each iteration one product)
-//   C = -Y U          right-hand-side terms (K, n)
-//   per iteration:    W = R M  (R = rho (z - u) - c) on fp16-split MFMA, then
-//                     the relaxation / projection / dual update of
-//                     nnls.py:152-157, which also writes the next R.
-// z and u are kept in FP64; products carry FP32-level rounding.
+//   C = -Y U          right-hand-side terms (K, n), FP64 MFMA GEMM
+//   per iteration:    W = W_a + (R - R_a) M  (R = rho (z - u)) on fp16-split
+//                     MFMA, then the relaxation / projection / dual update
+//                     of nnls.py:152-157, which also writes the next R - R_a;
+//   every 100:        the anchor R_a = R and W_a = M (R_a - c) in FP64 (FP64
+//                     MFMA GEMM) instead of the split product.
+// z and u are kept in FP64.  The reference's solve is (Q + rho I)^-1 (rho
+// (z - u) - c) (nnls.py:152-154).  The right-hand side is dominated by c,
+// and cond(Q + rho I) amplifies any rounding relative to it: taking the
+// anchor's part in FP64 leaves the split products (and their FP32
+// accumulation) only R's drift since the anchor to round.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -220,12 +226,14 @@ static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, con
 // ---------------------------------------------------------------------------
 // design matrix, Gram regularisation, trace
 // ---------------------------------------------------------------------------
-__global__ void to_float_kernel(const double* __restrict__ src, int64_t rows, int64_t cols,
-                                int64_t ld_dst, float* __restrict__ dst) {
+// dst[i][j] = src[i][j] * scale[j] for j < cols, 0 up to ld_dst (FP64).
+__global__ void scale_cols_kernel(const double* __restrict__ src, int64_t rows, int64_t cols,
+                                  const double* __restrict__ scale, int64_t ld_dst,
+                                  double* __restrict__ dst) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= rows * ld_dst) return;
   const int64_t i = idx / ld_dst, j = idx - i * ld_dst;
-  dst[idx] = (j < cols) ? static_cast<float>(src[i * cols + j]) : 0.f;
+  dst[idx] = (j < cols) ? src[i * cols + j] * scale[j] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -293,18 +301,26 @@ constexpr int AG_WAVES = 8;
 constexpr int AG_NT = 64 * AG_WAVES;            // its threads
 constexpr int AG_WCB = AG_BN / (32 * AG_WAVES); // 32-column blocks per wave
 static_assert(AG_WCB >= 1 && AG_WCB * 32 * AG_WAVES == AG_BN, "waves tile the 256 columns");
-constexpr float AG_TAU = 1.0f / 1024.0f;   // delta-product bound max|R - R_b| / max|R| per row
+constexpr float AG_TAU = 1.0f / 1024.0f;   // default delta-product bound max|R - R_b| / max|R| per row
+// Anchor refresh period of the ADMM loop.  A split product's rounding (the
+// fp16 hi / lo representation and the FP32 accumulation over n / S terms of
+// each split) is relative to the size of its operand.  The operand is
+// R - R_a, where the anchor R_a = rho (z - u) is taken every AG_ANCHOR
+// iterations and its product M (R_a - c) is formed in FP64 (FP64 MFMA
+// GEMM), so the rounding scales with R's drift since the anchor rather than
+// with R.  In those iterations the FP64 product replaces the split one.
+constexpr int AG_ANCHOR = 100;
 
 // Full product this iteration?  force > 0: yes; force < 0: no (timing of the
 // delta product only); otherwise yes when some row k < K has max|D_k| >
 // tau max|R_k| (D = R - R_b).  Block-uniform; every thread must call it.
 __device__ __forceinline__ bool admm_full_product(int force, int64_t K,
                                                   const unsigned* __restrict__ rmax,
-                                                  const unsigned* __restrict__ dmax) {
+                                                  const unsigned* __restrict__ dmax, float tau) {
   int need = force > 0 ? 1 : 0;
   if (force == 0)
     for (int64_t k = threadIdx.x; k < K; k += blockDim.x)
-      if (__uint_as_float(dmax[k]) > AG_TAU * __uint_as_float(rmax[k])) need = 1;
+      if (__uint_as_float(dmax[k]) > tau * __uint_as_float(rmax[k])) need = 1;
   return __syncthreads_or(need) != 0;
 }
 
@@ -496,14 +512,14 @@ template <int KC>
 __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
     int64_t K, int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
     const unsigned* __restrict__ rmax, const float* __restrict__ D,
-    const unsigned* __restrict__ dmax, int force, const _Float16* __restrict__ Mh,
-    const _Float16* __restrict__ Ml, float* __restrict__ part) {
+    const unsigned* __restrict__ dmax, const unsigned* __restrict__ fmax_, int force, float tau,
+    const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml, float* __restrict__ part) {
   constexpr int P = KC + 8;
   __shared__ __align__(16) _Float16 Bh[AG_BN][P];
   __shared__ __align__(16) _Float16 Bl[AG_BN][P];
   __shared__ __align__(16) _Float16 Ah[AG_KR][P];
   __shared__ __align__(16) _Float16 Al[AG_KR][P];
-  const bool full = admm_full_product(force, K, rmax, dmax);
+  const bool full = admm_full_product(force, K, fmax_, dmax, tau);
   // 1-D grid, row block of R fastest (the Kp/128 row blocks that share a tile
   // of M run back to back and share it through the caches), then column
   // block, then split; with one row block this is column block, then split.
@@ -558,29 +574,6 @@ __device__ __forceinline__ int64_t r_tile_index(int64_t k, int64_t i, int64_t Kp
   return ((i / AG_KP) * Kp + k) * AG_KP + i % AG_KP;
 }
 
-// init (nnls.py:122-125, 149): z = w_init, u = 0, R = rho (z - u) - c and the
-// row maxima of |R|.  Grid (ceil(n / 256), K): one row per blockIdx.y, one
-// atomic per block.  x is not written: the first update takes (z, u) =
-// (w_init, 0) from its arguments.
-__global__ __launch_bounds__(256) void admm_init2_kernel(int64_t K, int64_t n, int64_t Kp,
-                                                         double w_init,
-                                                         const double* __restrict__ rho_p,
-                                                         const float* __restrict__ c,
-                                                         float* __restrict__ R,
-                                                         unsigned* __restrict__ rowmax) {
-  __shared__ float red[4];
-  const int64_t k = blockIdx.y;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  float m = 0.f;
-  if (i < n) {
-    const float r = static_cast<float>(*rho_p * (w_init - 0.0) - static_cast<double>(c[k * n + i]));
-    R[r_tile_index(k, i, Kp)] = r;
-    m = fabsf(r);
-  }
-  m = block_max_256(m, red);
-  if (threadIdx.x == 0 && m > 0.f) atomicMax(rowmax + k, __float_as_uint(m));
-}
-
 __device__ __forceinline__ float block_max_256(float v, float* red) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -605,14 +598,14 @@ __device__ __forceinline__ float block_max_256(float v, float* red) {
 struct AdmmStep {
   double rho, alpha;
   // x <- w_hat + u_old with w_hat = alpha w + (1 - alpha) z_old; returns
-  // R = rho (z - u) - c of the new iterate and the step's |z - z_old|
-  __device__ __forceinline__ float apply(double w, double zo, double uo, float cv, double& x,
-                                         float& dz) const {
+  // rho (z - u) of the new iterate and the step's |z - z_old|
+  __device__ __forceinline__ double apply(double w, double zo, double uo, double& x,
+                                          float& dz) const {
     const double wh = alpha * w + (1.0 - alpha) * zo;
     x = wh + uo;
     const double zn = fmax(0.0, x), un = fmin(0.0, x);
     dz = static_cast<float>(fabs(zn - zo));
-    return static_cast<float>(rho * (zn - un) - static_cast<double>(cv));
+    return rho * (zn - un);
   }
 };
 
@@ -664,9 +657,9 @@ __device__ __forceinline__ void store_vec(double* __restrict__ p, const double (
 
 template <int V>
 __device__ __forceinline__ void sum_partials(const float* __restrict__ p, int64_t stride, int S,
-                                             float (&sum)[V]) {
+                                             double (&sum)[V]) {
 #pragma unroll
-  for (int e = 0; e < V; ++e) sum[e] = 0.f;
+  for (int e = 0; e < V; ++e) sum[e] = 0.0;
   int s = 0;
   for (; s + 4 <= S; s += 4) {
     float a[4][V];
@@ -675,52 +668,61 @@ __device__ __forceinline__ void sum_partials(const float* __restrict__ p, int64_
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int e = 0; e < V; ++e) sum[e] += a[q][e];
+      for (int e = 0; e < V; ++e) sum[e] += static_cast<double>(a[q][e]);
   }
   for (; s < S; ++s) {
     float a[V];
     load_vec<V>(p + (int64_t)s * stride, a);
 #pragma unroll
-    for (int e = 0; e < V; ++e) sum[e] += a[e];
+    for (int e = 0; e < V; ++e) sum[e] += static_cast<double>(a[e]);
   }
 }
 
 // Per-iteration buffers of the ADMM loop (all device pointers).
 struct AdmmIter {
-  float* R;                 // Kp x npad blocked: this iteration's R, replaced by the next
-  float* Rb;                // Kp x npad blocked: R at the last full product
-  float* D;                 // Kp x npad blocked: next R - R_b
-  float* wb;                // K x n: w of the last full product
-  const unsigned* rcur;     // max |R_k| of this iteration's R
+  float* R;                 // Kp x npad blocked: this iteration's operand R - R_a
+  float* Rb;                // Kp x npad blocked: the operand at the last full product
+  float* D;                 // Kp x npad blocked: next operand - R_b
+  double* wb;               // K x n: the product of the last full product
+  const double* Ra;         // K x n: the anchor R_a
+  const unsigned* rcur;     // max |R_k - R_a,k| of this iteration's operand (split scales)
   unsigned* rnext;
   unsigned* rclear;
   const unsigned* dcur;     // max |D_k| of this iteration's D
   unsigned* dnext;
   unsigned* dclear;
+  const unsigned* fcur;     // max |R_k| = max rho |x_k| (the delta test's reference)
+  unsigned* fnext;
+  unsigned* fclear;
   unsigned* full_count;     // full products taken
 };
 
+// anchor == 1: this iteration's product was not run (the anchor was just
+// refreshed, so the operand R - R_a is zero): the partial sums count as 0 and
+// the iteration counts as a full product (new base R_b = 0, w_b = 0).
 template <int V>
 __global__ __launch_bounds__(256) void admm_update_kernel(
     int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
-    const double* __restrict__ rho_p, const float* __restrict__ c, double* __restrict__ x,
-    int first, int force, double w_init, AdmmIter b, double alpha, const int* __restrict__ active,
-    unsigned* __restrict__ delta) {
+    const double* __restrict__ rho_p, const double* __restrict__ wc, double* __restrict__ x,
+    int first, int force, int anchor, float tau, double w_init, AdmmIter b, double alpha,
+    const int* __restrict__ active, unsigned* __restrict__ delta) {
   __shared__ float red[4];
-  const bool full = admm_full_product(force, K, b.rcur, b.dcur);
+  const bool full = admm_full_product(force, K, b.fcur, b.dcur, tau);
   const int64_t k = blockIdx.y;
   const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * V;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     b.rclear[k] = 0u;
     b.dclear[k] = 0u;
+    b.fclear[k] = 0u;
     if (k == K - 1)
       for (int64_t q = K; q < Kp; ++q) {
         b.rclear[q] = 0u;
         b.dclear[q] = 0u;
+        b.fclear[q] = 0u;
       }
-    if (k == 0 && full) *b.full_count += 1u;
+    if (k == 0 && full && !anchor) *b.full_count += 1u;
   }
-  float rmax = 0.f, dmx = 0.f, zmv = 0.f;
+  float rmax = 0.f, dmx = 0.f, fmx = 0.f, zmv = 0.f;
   if (i0 < n) {
     const int64_t ri = r_tile_index(k, i0, Kp);
     float rcur[V], rb[V], rn[V], dn[V];
@@ -734,35 +736,39 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
     } else {
       load_vec<V>(b.Rb + ri, rb);
     }
+    const int64_t idx = k * n + i0;
     if (!active[k]) {
+      double xo[V];
+      load_vec<V>(x + idx, xo);
 #pragma unroll
-      for (int e = 0; e < V; ++e) rn[e] = rcur[e];
+      for (int e = 0; e < V; ++e) {
+        rn[e] = rcur[e];
+        fmx = fmaxf(fmx, static_cast<float>(*rho_p * fabs(xo[e])));
+      }
     } else {
-      const int64_t idx = k * n + i0;
       const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
       const float sA = pow2_scale_for(__uint_as_float(full ? b.rcur[k] : b.dcur[k]), 14);
       const double inv = 1.0 / (static_cast<double>(sM) * static_cast<double>(sA));
       const AdmmStep st{*rho_p, alpha};
-      float sum[V];
-      sum_partials<V>(part + k * npad + i0, Kp * npad, S, sum);
+      double sum[V];
+      if (anchor) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) sum[e] = 0.0;
+      } else {
+        sum_partials<V>(part + k * npad + i0, Kp * npad, S, sum);
+      }
       double w[V], xo[V], xn[V];
-      float cv[V];
-      load_vec<V>(c + idx, cv);
-      // w_b in FP32 holds the full product's w exactly: an FP32 sum times a
-      // power of two
-      float wf[V];
+      // the split partials are added in FP64; w_b keeps the full product
       if (full) {
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          wf[e] = sum[e] * static_cast<float>(inv);
-          w[e] = static_cast<double>(wf[e]);
+          const double p = sum[e] * inv;
+          b.wb[idx + e] = p;
+          w[e] = wc[idx + e] + p;
         }
-        store_vec<V>(b.wb + idx, wf);
       } else {
-        load_vec<V>(b.wb + idx, wf);
 #pragma unroll
-        for (int e = 0; e < V; ++e)
-          w[e] = static_cast<double>(wf[e]) + static_cast<double>(sum[e]) * inv;
+        for (int e = 0; e < V; ++e) w[e] = wc[idx + e] + (b.wb[idx + e] + sum[e] * inv);
       }
       if (first) {
 #pragma unroll
@@ -775,7 +781,9 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
         const double zo = first ? w_init : fmax(0.0, xo[e]);
         const double uo = first ? 0.0 : fmin(0.0, xo[e]);
         float dz;
-        rn[e] = st.apply(w[e], zo, uo, cv[e], xn[e], dz);
+        const double rf = st.apply(w[e], zo, uo, xn[e], dz);
+        rn[e] = static_cast<float>(rf - b.Ra[idx + e]);
+        fmx = fmaxf(fmx, static_cast<float>(fabs(rf)));
         zmv = fmaxf(zmv, dz);
       }
       store_vec<V>(x + idx, xn);
@@ -793,26 +801,53 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
   __syncthreads();
   dmx = block_max_256(dmx, red);
   __syncthreads();
+  fmx = block_max_256(fmx, red);
+  __syncthreads();
   zmv = block_max_256(zmv, red);
   if (threadIdx.x == 0) {
     if (rmax > 0.f) atomicMax(b.rnext + k, __float_as_uint(rmax));
     if (dmx > 0.f) atomicMax(b.dnext + k, __float_as_uint(dmx));
+    if (fmx > 0.f) atomicMax(b.fnext + k, __float_as_uint(fmx));
     if (zmv > 0.f) atomicMax(delta + k, __float_as_uint(zmv));
   }
+}
+
+// Anchor refresh (before the product of an iteration): R_a = rho (z - u) =
+// rho |x| of the current iterate (rho w_init before the first), V = R_a - c
+// for the FP64 product W_a = M V that follows, and the operand R - R_a = 0.
+// Grid (ceil(n / 256), K).
+__global__ __launch_bounds__(256) void admm_anchor_kernel(int64_t K, int64_t n, int64_t Kp,
+                                                          int first, double w_init,
+                                                          const double* __restrict__ rho_p,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ C,
+                                                          double* __restrict__ Ra,
+                                                          double* __restrict__ Va,
+                                                          float* __restrict__ R) {
+  const int64_t k = blockIdx.y;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t idx = k * n + i;
+  const double rf = *rho_p * (first ? w_init : fabs(x[idx]));
+  Ra[idx] = rf;
+  Va[idx] = rf - C[idx];
+  R[r_tile_index(k, i, Kp)] = 0.f;
 }
 
 static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct NnlsWs {
-  float* Ut;      // n x ldu
   _Float16* Uh;   // gpad x mpad, fp16 hi plane of Ut (row i scaled by us[i]), Gram-blocked
   _Float16* Ul;   // gpad x mpad, fp16 lo plane
   float* us;      // gpad power-of-two row scales
   int64_t gpad, mpad;
   double* Q64;    // n x n FP64, inverted in place
   double* work64; // n*n + n work of the recursive inverse
-  float* Yf;      // K x ldu
-  float* C;       // K x n
+  double* Ys;     // K x ldu: Y times the design rows' scales
+  double* C;      // K x n: c = -Y U (nnls.py:143)
+  double* Wc;     // K x n: W_a = M (R_a - c), the solve's FP64 part
+  double* Ra;     // K x n: the anchor R_a = rho (z - u) at the last refresh
+  double* Va;     // K x n: R_a - c
   double* x;      // K x n ADMM iterate, z = max(0, x), u = min(0, x)
   float* rho;     // 1
   double* rho64;  // 1
@@ -824,10 +859,11 @@ struct NnlsWs {
   float* Rp;        // Kp x npad, blocked (r_tile_index)
   float* Rb;        // Kp x npad, blocked: R at the last full product
   float* D;         // Kp x npad, blocked: R - Rb
-  float* wb;        // K x n: w of the last full product
+  double* wb;       // K x n: the product of the last full product
   unsigned* full_count;  // full ADMM products taken by the last solve
   float* part;      // S x Kp x npad
-  unsigned* rmax;   // 6 x Kp: three rotating row-maximum sets of |R|, then three of |D|
+  unsigned* rmax;   // 9 x Kp: three rotating row-maximum sets of the operand |R - R_a|,
+                    // three of |D|, three of |R| = rho |x|
   int64_t npad, Kp;
   int S;
   int64_t bytes;
@@ -856,16 +892,23 @@ static int admm_splits_for(int64_t npad, int64_t Kp, int occ) {
 }
 
 // One ADMM product launch (admm_gemm_kernel) over S splits of the reduction.
+// rmax: the operand's row maxima (split scales of a full product); fmax_:
+// max rho |x_k|, against which tau tests dmax (unused when force != 0).
 static void launch_admm_product(hipStream_t st, int64_t K, int64_t Kp, int64_t npad, int S,
                                 const float* R, const unsigned* rmax, const float* D,
-                                const unsigned* dmax, int force, const _Float16* Mh,
-                                const _Float16* Ml, float* part) {
+                                const unsigned* dmax, const unsigned* fmax_, int force, float tau,
+                                const _Float16* Mh, const _Float16* Ml, float* part) {
   const int64_t nch = npad / AG_KP;
   const int cps = static_cast<int>((nch + S - 1) / S);
   const dim3 grid((unsigned)((npad / AG_BN) * S * (Kp / AG_KR)));
   hipLaunchKernelGGL((admm_gemm_kernel<AG_KP>), grid, dim3(AG_NT), 0, st, K, Kp, npad, cps, R, rmax,
-                     D, dmax, force, Mh, Ml, part);
+                     D, dmax, fmax_, force, tau, Mh, Ml, part);
 }
+
+// Delta-product bound tau (the product's accuracy / speed knob, process-wide
+// like the occupancy): an iteration takes the delta product while every row
+// satisfies max|R_k - R_b,k| <= tau max|R_k|.  0 makes every product full.
+static std::atomic<float> g_admm_tau{AG_TAU};
 
 static int admm_splits(int64_t npad, int64_t Kp) {
   return admm_splits_for(npad, Kp, g_admm_occ.load(std::memory_order_relaxed));
@@ -881,7 +924,6 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
     off += align_up(bytes, 256);
     return q;
   };
-  w.Ut = reinterpret_cast<float*>(take(n * ldu * 4));
   w.gpad = align_up(n, 256);
   w.mpad = align_up(m, 32);
   w.Uh = reinterpret_cast<_Float16*>(take(w.gpad * w.mpad * 2));
@@ -889,8 +931,11 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.us = reinterpret_cast<float*>(take(w.gpad * 4));
   w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
   w.work64 = reinterpret_cast<double*>(take((n * n + n) * 8));
-  w.Yf = reinterpret_cast<float*>(take(K * ldu * 4));
-  w.C = reinterpret_cast<float*>(take(K * n * 4));
+  w.Ys = reinterpret_cast<double*>(take(K * ldu * 8));
+  w.C = reinterpret_cast<double*>(take(K * n * 8));
+  w.Wc = reinterpret_cast<double*>(take(K * n * 8));
+  w.Ra = reinterpret_cast<double*>(take(K * n * 8));
+  w.Va = reinterpret_cast<double*>(take(K * n * 8));
   w.x = reinterpret_cast<double*>(take(K * n * 8));
   w.rho = reinterpret_cast<float*>(take(4));
   w.rho64 = reinterpret_cast<double*>(take(8));
@@ -905,10 +950,10 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Rp = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.part = reinterpret_cast<float*>(
       take((int64_t)admm_splits_for(w.npad, w.Kp, AG_OCC_MAX) * w.Kp * w.npad * 4));
-  w.rmax = reinterpret_cast<unsigned*>(take(6 * w.Kp * 4));
+  w.rmax = reinterpret_cast<unsigned*>(take(9 * w.Kp * 4));
   w.Rb = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
-  w.wb = reinterpret_cast<float*>(take(K * n * 4));
+  w.wb = reinterpret_cast<double*>(take(K * n * 8));
   w.full_count = reinterpret_cast<unsigned*>(take(4));
   w.bytes = off;
   return w;
@@ -945,11 +990,11 @@ __device__ __forceinline__ int64_t g_tile_index(int64_t i, int64_t k, int64_t nc
   return ((i / GR_TM * nchk + k / AG_KP) * GR_TM + i % GR_TM) * AG_KP + k % AG_KP;
 }
 
-// One workgroup per (padded) row i: Ut[i, :] = lib[i, :m] * scale (FP32, for
-// the right-hand sides) and its scaled fp16 hi / lo planes.
+// One workgroup per (padded) row i: the scaled fp16 hi / lo planes of
+// U^T[i, :] = lib[i, :m] * scale.
 __global__ __launch_bounds__(256) void build_ut_split_kernel(
-    const double* __restrict__ lib, int64_t ld_lib, int64_t n, int64_t m, int64_t ldu,
-    int64_t mpad, const double* __restrict__ scale, float* __restrict__ Ut,
+    const double* __restrict__ lib, int64_t ld_lib, int64_t n, int64_t m,
+    int64_t mpad, const double* __restrict__ scale,
     _Float16* __restrict__ Uh, _Float16* __restrict__ Ul, float* __restrict__ us) {
   __shared__ float red[4];
   const int64_t i = blockIdx.x;
@@ -963,7 +1008,6 @@ __global__ __launch_bounds__(256) void build_ut_split_kernel(
   if (threadIdx.x == 0) us[i] = sc;
   for (int64_t j = threadIdx.x; j < mpad; j += blockDim.x) {
     const double u = (i < n && j < m) ? lib[i * ld_lib + j] * scale[j] : 0.0;
-    if (i < n && j < ldu) Ut[i * ldu + j] = static_cast<float>(u);
     const double v = u * static_cast<double>(sc);
     const _Float16 h = static_cast<_Float16>(v);
     const int64_t d = g_tile_index(i, j, nchk);
@@ -1007,12 +1051,24 @@ static int64_t gram_tiles(int64_t gpad) {
 
 static int64_t gram_grid(int64_t gpad) { return 8 * ((gram_tiles(gpad) + 7) / 8); }
 
+// Segments of the Gram's reduction accumulated in FP32: each of GR_FLUSH
+// launches covers 1/GR_FLUSH of the m products of every entry and adds its
+// FP32 sums into the FP64 Q (the first stores them).  One FP32 accumulation
+// over all m = 17,288 design rows of plain-20k moves the NNLS weights by
+// 1.7e-4 from an FP64 Gram (the library's conditioning amplifies it); 8
+// segments keep that near 1e-5 for about two extra reads and writes of
+// the lower triangle of Q.
+#ifndef GR_FLUSH
+#define GR_FLUSH 8
+#endif
+
 // One (128 x 256) tile of Q per workgroup (4 waves of 128 x 64), 32-index
-// chunks staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep step; writes
-// Q[i][j] for j <= i < n.
+// chunks ch0 .. ch1 - 1 staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep
+// step; stores (first) or adds the segment's sums to Q[i][j] for j <= i < n.
 __global__ __launch_bounds__(GR_NT, AG_OCC) void gram_split_kernel(
     int64_t n, int64_t gpad, int64_t mpad, const _Float16* __restrict__ Uh,
-    const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q) {
+    const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q,
+    int64_t ch0, int64_t ch1, int first) {
   constexpr int P = AG_KP + 8;
   __shared__ __align__(16) _Float16 Bh[2 * GR_TM][P];
   __shared__ __align__(16) _Float16 Bl[2 * GR_TM][P];
@@ -1035,7 +1091,7 @@ __global__ __launch_bounds__(GR_NT, AG_OCC) void gram_split_kernel(
     for (int b = 0; b < GR_WCB; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  for (int64_t ch = 0; ch < nchk; ++ch) {
+  for (int64_t ch = ch0; ch < ch1; ++ch) {
     // stage chunk ch: each plane tile is 512 16-byte runs, thread t moves runs t + GR_NT h
     const int64_t oa = ((tileA + ch) * GR_TM) * AG_KP;
     const int64_t ob0 = ((tileB0 + ch) * GR_TM) * AG_KP;
@@ -1084,8 +1140,11 @@ __global__ __launch_bounds__(GR_NT, AG_OCC) void gram_split_kernel(
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t i = i0 + rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
-        if (i < n && j <= i)
-          Q[i * n + j] = static_cast<double>(acc[rb][cb][r]) / (static_cast<double>(us[i]) * sj);
+        if (i < n && j <= i) {
+          const double v = static_cast<double>(acc[rb][cb][r]) / (static_cast<double>(us[i]) * sj);
+          double* q = Q + i * n + j;
+          *q = first ? v : *q + v;
+        }
       }
     }
 }
@@ -1354,10 +1413,15 @@ static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64
                        int64_t n, int64_t m, const double* row_scale, double lambda_reg) {
   const int64_t ldu = align_up(m, 16);
   hipLaunchKernelGGL(build_ut_split_kernel, dim3((unsigned)w.gpad), dim3(256), 0, st, lib, ld_lib,
-                     n, m, ldu, w.mpad, row_scale, w.Ut, w.Uh, w.Ul, w.us);
+                     n, m, w.mpad, row_scale, w.Uh, w.Ul, w.us);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(GR_NT), 0, st, n,
-                     w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64);
+  const int64_t nchk = w.mpad / AG_KP;
+  const int64_t seg = (nchk + GR_FLUSH - 1) / GR_FLUSH;
+  for (int64_t ch0 = 0; ch0 < nchk; ch0 += seg) {
+    const int64_t ch1 = ch0 + seg < nchk ? ch0 + seg : nchk;
+    hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(GR_NT), 0, st, n,
+                       w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64, ch0, ch1, ch0 == 0 ? 1 : 0);
+  }
   SMX_LAUNCHED();
   hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
                      lambda_reg / static_cast<double>(n), w.rho64, w.rho);
@@ -1395,11 +1459,12 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
   stage_end(ST_FACTOR, st, 1);
   stage_begin(ST_RHS, st);
-  // C = -Y U
-  hipLaunchKernelGGL(to_float_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m, ldu,
-                     w.Yf);
+  // C = -Y U in FP64 (FP64 MFMA): U[j][i] = lib[i][j] row_scale[j], so
+  // C[k][i] = -sum_j (Y[k][j] row_scale[j]) lib[i][j] (nnls.py:143)
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m,
+                     row_scale, ldu, w.Ys);
   SMX_LAUNCHED();
-  if (int e = gemm_nt<float>(st, K, n, ldu, -1.f, w.Yf, ldu, w.Ut, ldu, 0.f, w.C, n)) return e;
+  if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, lib, ld_lib, 0.0, w.C, n)) return e;
   const int64_t tot = K * n;
   const int64_t npad = w.npad, Kp = w.Kp;
   const int S = w.S;
@@ -1409,14 +1474,11 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   for (float* buf : {w.Rp, w.Rb, w.D})
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st,
                        reinterpret_cast<unsigned*>(buf), Kp * npad, 0u);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(6 * Kp, 256)), dim3(256), 0, st, w.rmax,
-                     6 * Kp, 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(9 * Kp, 256)), dim3(256), 0, st, w.rmax,
+                     9 * Kp, 0u);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, w.full_count, 1, 0u);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(admm_init2_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st, K,
-                     n, Kp, w_init, w.rho64, w.C, w.Rp, w.rmax);
-  SMX_LAUNCHED();
-  stage_end(ST_RHS, st, 5);
+  stage_end(ST_RHS, st, 4);
   std::vector<int> act(K, 1);
   SMX_HIP(hipMemcpyAsync(w.active, act.data(), K * sizeof(int), hipMemcpyHostToDevice, st));
   const int check = 25;
@@ -1438,24 +1500,38 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     ib.dcur = w.rmax + (3 + it % 3) * Kp;
     ib.dnext = w.rmax + (3 + (it + 1) % 3) * Kp;
     ib.dclear = w.rmax + (3 + (it + 2) % 3) * Kp;
+    ib.fcur = w.rmax + (6 + it % 3) * Kp;
+    ib.fnext = w.rmax + (6 + (it + 1) % 3) * Kp;
+    ib.fclear = w.rmax + (6 + (it + 2) % 3) * Kp;
+    ib.Ra = w.Ra;
     ib.full_count = w.full_count;
     // the first product has no base; the last is full so that the result
-    // carries no delta error
-    const int force = (it == 0 || it == maxiter - 1) ? 1 : 0;
+    // carries no delta error.  An anchor iteration replaces its product by
+    // the FP64 one: the operand is zero and w = W_a exactly.
+    const int anchor = (it % AG_ANCHOR == 0) ? 1 : 0;
+    const int force = (anchor || it == maxiter - 1) ? 1 : 0;
     const int first = it == 0 ? 1 : 0;
-    stage_begin(ST_ADMM_PRODUCT, st);
-    launch_admm_product(st, K, Kp, npad, S, w.Rp, ib.rcur, w.D, ib.dcur, force, w.Mh, w.Ml,
-                        w.part);
-    SMX_LAUNCHED();
-    stage_end(ST_ADMM_PRODUCT, st, 1);
+    const float tau = g_admm_tau.load(std::memory_order_relaxed);
+    if (anchor) {
+      hipLaunchKernelGGL(admm_anchor_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
+                         K, n, Kp, first, w_init, w.rho64, w.x, w.C, w.Ra, w.Va, w.Rp);
+      SMX_LAUNCHED();
+      if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n)) return e;
+    } else {
+      stage_begin(ST_ADMM_PRODUCT, st);
+      launch_admm_product(st, K, Kp, npad, S, w.Rp, ib.rcur, w.D, ib.dcur, ib.fcur, force, tau,
+                          w.Mh, w.Ml, w.part);
+      SMX_LAUNCHED();
+      stage_end(ST_ADMM_PRODUCT, st, 1);
+    }
     if (n % 4 == 0)
       hipLaunchKernelGGL(admm_update_kernel<4>, dim3(blocks_for(n / 4, 256), (unsigned)K), dim3(256),
-                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.x, first, force, w_init,
-                         ib, 1.6, w.active, w.delta);
+                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.Wc, w.x, first, force, anchor,
+                         tau, w_init, ib, 1.6, w.active, w.delta);
     else
       hipLaunchKernelGGL(admm_update_kernel<1>, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0,
-                         st, K, n, Kp, npad, S, w.part, w.rho64, w.C, w.x, first, force, w_init, ib,
-                         1.6, w.active, w.delta);
+                         st, K, n, Kp, npad, S, w.part, w.rho64, w.Wc, w.x, first, force, anchor,
+                         tau, w_init, ib, 1.6, w.active, w.delta);
     SMX_LAUNCHED();
     done = it + 1;
     if (chk) {
@@ -1559,6 +1635,18 @@ int smx_admm_set_occupancy(int32_t workgroups_per_cu) {
   return 0;
 }
 
+int smx_admm_set_delta_threshold(double tau) {
+  SMX_REQUIRE(tau >= 0.0 && tau <= 0.25, "smx_admm_set_delta_threshold: tau must be in [0, 0.25]");
+  g_admm_tau.store(static_cast<float>(tau), std::memory_order_relaxed);
+  return 0;
+}
+
+int smx_admm_get_delta_threshold(double* tau) {
+  SMX_REQUIRE(tau, "smx_admm_get_delta_threshold: null pointer");
+  *tau = static_cast<double>(g_admm_tau.load(std::memory_order_relaxed));
+  return 0;
+}
+
 int smx_admm_get_occupancy(int32_t* workgroups_per_cu) {
   SMX_REQUIRE(workgroups_per_cu, "smx_admm_get_occupancy: null pointer");
   *workgroups_per_cu = g_admm_occ.load(std::memory_order_relaxed);
@@ -1577,7 +1665,8 @@ int smx_admm_product_time(int64_t n, int64_t m, int64_t K, void* workspace, int3
   SMX_HIP(hipEventCreate(&e1));
   SMX_HIP(hipEventRecord(e0, st));
   for (int r = 0; r < reps; ++r)
-    launch_admm_product(st, K, Kp, npad, w.S, w.Rp, w.rmax, w.D, w.rmax + 3 * Kp, full ? 1 : -1,
+    launch_admm_product(st, K, Kp, npad, w.S, w.Rp, w.rmax, w.D, w.rmax + 3 * Kp, w.rmax,
+                        full ? 1 : -1, AG_TAU,
                         w.Mh, w.Ml, w.part);
   SMX_LAUNCHED();
   SMX_HIP(hipEventRecord(e1, st));
@@ -1625,7 +1714,9 @@ static int operator_apply(int64_t n, int64_t m, int64_t K_max, void* workspace, 
   SMX_LAUNCHED();
   stage_begin(ST_ADMM_PRODUCT, st);
   // the padded rows serve as R (full) or as D (delta), with the same row maxima
-  launch_admm_product(st, K, Kp, npad, S, w.Rp, w.rmax, w.Rp, w.rmax, hi_only ? -1 : 1, w.Mh, w.Ml,
+  launch_admm_product(st, K, Kp, npad, S, w.Rp, w.rmax, w.Rp, w.rmax, w.rmax, hi_only ? -1 : 1,
+                      AG_TAU,
+                      w.Mh, w.Ml,
                       w.part);
   SMX_LAUNCHED();
   stage_end(ST_ADMM_PRODUCT, st, 1);

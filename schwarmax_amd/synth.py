@@ -1,6 +1,6 @@
 """Seeded synthetic galaxies with the shapes of the benchmark workloads.
 
-Host-side NumPy only: builds ``dict_data`` in the layout the reference's loader
+Builds ``dict_data`` in the layout the reference's loader
 produces (utils.py:572-743) — Voronoi apertures on a pixel grid with a ``-1``
 sentinel after the last pixel, the (R, z, phi) cell-centre arrays sorted by
 cell index, the unscrambled 1024-point Sobol set, and bootstrap normals drawn
@@ -12,7 +12,9 @@ seeded NumPy generator and carried in ``dict_data['jeans_normals']`` and
 without those keys the model takes the reference's own streams.
 
 Observations (surface density and h1..h4) are made by ``observe`` from an orbit
-library and a seeded Dirichlet weight vector, plus Gaussian noise.
+library and a seeded Dirichlet weight vector, plus Gaussian noise.  Everything
+here is host NumPy except ``observed_workload``, which builds the library it
+observes on the GPU.
 """
 import math
 
@@ -241,3 +243,39 @@ def observe(dd, A_xy, A_h, mmpo, M_to_L, seed=1, noise=0.02, h_err=0.03, floor=N
     dd['V_data_err'] = np.full(B, 5.0)
     dd['sigma_data_err'] = np.full(B, 5.0)
     return w
+
+
+def observed_workload(config, seed, n=None, return_library=False):
+    """A galaxy of a benchmark configuration with observations, on the GPU.
+
+    The initial conditions and the orbit library come from the device path
+    (smx_initial_conditions, smx_build_library); (v0, s) are fitted to the
+    tracers' line-of-sight velocities and the observations are made from
+    that library by ``observe`` (surface density floored at 1e-3 of its median).
+    Returns (theta, dict_data, schedule, info) with info = {'P', 'B',
+    'valid'} (the valid fraction of the integrations) and, when
+    ``return_library``, 'lib' (the device library, (n, P + 6B) FP64)."""
+    import torch
+    from . import model as M, likelihood as L, potentials as POT
+    theta, dd, sched = make_galaxy(config, seed=seed, n=n)
+    ph, pb = L.theta_to_params(theta)
+    s = M.schedule_of(sched)
+    pot = POT.pack_potential(M.baryon_params(pb), ph)
+    wb, Tt, dt0 = M.initial_conditions(pot, dd, s['n_realizations'], s)
+    rot = M.makeRotationMatrix(pb['alpha'], pb['beta'], pb['gamma'])
+    fit_base_gaussians(dd, wb[:, 0, :].cpu().numpy(), rot)
+    Rl, Rn, xyl, xyn = L.grid_args(dd)
+    g = M.make_grids(int(dd['total_bins']), Rn, Rl, xyl, xyn, rot, True)
+    libd = M.build_library(pot, g, M.make_schedule(s, -pb['Omega_bar']), wb, Tt, dt0, dd)
+    lib = libd['lib'].cpu().numpy()
+    P, B = libd['P'], libd['B']
+    A_xy = lib[:, P:P + B].T
+    A_h = [lib[:, P + B * k:P + B * (k + 1)].T for k in (1, 2, 3, 4)]
+    y_R = M.density_target(pb, dd).cpu().numpy()
+    observe(dd, A_xy, A_h, y_R.sum() / lib.shape[0], pb['mass_to_light_ratio'], floor=1e-3)
+    info = {'P': P, 'B': B, 'valid': float(libd['valid'].mean().item())}
+    if return_library:
+        info['lib'] = libd['lib']
+    del libd, wb, Tt, dt0
+    torch.cuda.empty_cache()
+    return theta, dd, sched, info

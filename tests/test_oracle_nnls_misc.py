@@ -92,3 +92,21 @@ def test_jeans_sigma_z_against_quad(canon_params):
         # 1000-point trapezoid vs adaptive quadrature: discretisation-level agreement
         assert abs(sz[i] ** 2 - I / nu) < 1e-4 * abs(I / nu)
         assert sR[i] == sz[i] and sphi[i] == sz[i]
+
+
+def test_fp64_torch_admm_equals_oracle():
+    """The torch FP64 restatement used at full shape on the GPU reproduces the
+    oracle's iterate sequence (nnls.py:101-163) on a small problem."""
+    import torch
+    from oracle import nnls as ON
+    from fp64_admm import admm_fp64
+    rng = np.random.default_rng(7)
+    m, n, K = 90, 60, 5
+    U = rng.random((m, n)) * (rng.random((m, n)) < 0.3)
+    Y = rng.random((K, m)) * 3.0
+    Q, rho, L = ON.gram_factor(U, 1.0)
+    ref = ON.admm(L, rho, -(Y @ U), 0.7, 300)
+    got = admm_fp64(torch.tensor(U), torch.tensor(Y), 1.0, 300, 0.7).numpy()
+    assert got.shape == ref.shape == (K, n)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    assert np.any(ref == 0.0) and np.any(ref > 0.0)            # the projection is active

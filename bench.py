@@ -373,7 +373,9 @@ def main():
     # algorithmic bytes per launch: both fp16 planes of M (4 n^2) on a full
     # product, the hi plane (2 n^2) on a delta product; the operand R or D in
     # fp16 hi + lo (4 K n) and the FP32 product out (4 K n)
-    n_prod = max(NN.STATS['products'], 1)
+    # launches: every ADMM iteration but the anchor refreshes (every 100th),
+    # whose product is an FP64 GEMM instead
+    n_prod = max(stages['admm_product'][1], 1)
     full_frac = NN.STATS['full_products'] / n_prod
     prod_bytes = (4.0 * full_frac + 2.0 * (1.0 - full_frac)) * n * n + 4.0 * K * n + 4.0 * K * n
     if stages['integrate'][0] >= stages['admm_product'][0]:

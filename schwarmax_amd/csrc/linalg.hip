@@ -138,12 +138,22 @@ struct NtLoader {
   }
 };
 
+// kchunk > 0: split-K, block z reduces indices [z kchunk, (z + 1) kchunk) and
+// writes its partial product to C + z split_stride (beta must be 0).
 template <class T, bool VEC>
 __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(int M, int N, int K, T alpha,
                                                          const T* __restrict__ A, int64_t lda,
                                                          const T* __restrict__ B, int64_t ldb, T beta,
-                                                         T* __restrict__ C, int64_t ldc, int lower_only) {
+                                                         T* __restrict__ C, int64_t ldc, int lower_only,
+                                                         int64_t kchunk, int64_t split_stride) {
   typedef NtMfma<T> MM;
+  if (kchunk > 0) {
+    const int64_t kz = (int64_t)blockIdx.z * kchunk;
+    A += kz;
+    B += kz;
+    C += (int64_t)blockIdx.z * split_stride;
+    K = static_cast<int>(K - kz < kchunk ? K - kz : kchunk);
+  }
   constexpr int PITCH = NT_TM + (sizeof(T) == 8 ? 2 : 4);
   __shared__ T As[NT_TK][PITCH];
   __shared__ T Bs[NT_TK][PITCH];
@@ -205,20 +215,60 @@ __global__ __launch_bounds__(256, 2) void gemm_nt_kernel(int M, int N, int K, T 
       }
 }
 
+// C[i][j] = sum_z P[z][i][j] (+ beta C[i][j]) over the split-K partials.
+template <class T>
+__global__ void split_sum_kernel(int64_t M, int64_t N, int S, const T* __restrict__ P, T beta,
+                                 T* __restrict__ C, int64_t ldc) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t i = idx / N, j = idx - i * N;
+  T acc = (T)0;
+  for (int z = 0; z < S; ++z) acc += P[(int64_t)z * M * N + idx];
+  T* p = C + i * ldc + j;
+  *p = (beta == (T)0) ? acc : acc + beta * (*p);
+}
+
+// work (optional, work_elems elements): when the output has fewer 128 x 128
+// tiles than the chip has CUs, the reduction is split over up to 8 blocks
+// of workgroups whose partials go to work and are summed in T.
 template <class T>
 static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
-                   const T* B, int64_t ldb, T beta, T* C, int64_t ldc, int lower_only = 0) {
+                   const T* B, int64_t ldb, T beta, T* C, int64_t ldc, int lower_only = 0,
+                   T* work = nullptr, int64_t work_elems = 0) {
   if (M <= 0 || N <= 0) return 0;
   constexpr int V = 16 / (int)sizeof(T);
   const bool vec = (lda % V == 0) && (ldb % V == 0) && (K % V == 0) &&
                    (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
   dim3 grid((unsigned)((N + NT_TN - 1) / NT_TN), (unsigned)((M + NT_TM - 1) / NT_TM));
+  int64_t S = 1, kchunk = 0;
+  const int64_t tiles = (int64_t)grid.x * grid.y;
+  if (work && !lower_only && tiles < 256) {
+    S = (256 + tiles - 1) / tiles;
+    if (S > 8) S = 8;
+    kchunk = ((K + S - 1) / S + NT_TK - 1) / NT_TK * NT_TK;
+    S = (K + kchunk - 1) / kchunk;
+    if (S < 2 || S * M * N > work_elems) S = 1;
+  }
+  T* out = C;
+  int64_t ld_out = ldc;
+  T b = beta;
+  if (S > 1) {
+    grid.z = (unsigned)S;
+    out = work;
+    ld_out = N;
+    b = (T)0;
+  } else {
+    kchunk = 0;
+  }
   if (vec)
     hipLaunchKernelGGL((gemm_nt_kernel<T, true>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, alpha,
-                       A, lda, B, ldb, beta, C, ldc, lower_only);
+                       A, lda, B, ldb, b, out, ld_out, lower_only, kchunk, M * N);
   else
     hipLaunchKernelGGL((gemm_nt_kernel<T, false>), grid, dim3(256), 0, st, (int)M, (int)N, (int)K, alpha,
-                       A, lda, B, ldb, beta, C, ldc, lower_only);
+                       A, lda, B, ldb, b, out, ld_out, lower_only, kchunk, M * N);
+  if (S > 1)
+    hipLaunchKernelGGL(split_sum_kernel<T>, dim3(blocks_for(M * N, 256)), dim3(256), 0, st, M, N,
+                       (int)S, work, beta, C, ldc);
   SMX_LAUNCHED();
   return 0;
 }
@@ -1464,7 +1514,9 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m,
                      row_scale, ldu, w.Ys);
   SMX_LAUNCHED();
-  if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, lib, ld_lib, 0.0, w.C, n)) return e;
+  if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, lib, ld_lib, 0.0, w.C, n, 0, w.work64,
+                              n * n + n))
+    return e;
   const int64_t tot = K * n;
   const int64_t npad = w.npad, Kp = w.Kp;
   const int S = w.S;
@@ -1516,7 +1568,9 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
       hipLaunchKernelGGL(admm_anchor_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
                          K, n, Kp, first, w_init, w.rho64, w.x, w.C, w.Ra, w.Va, w.Rp);
       SMX_LAUNCHED();
-      if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n)) return e;
+      if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n, 0, w.work64,
+                                  n * n + n))
+        return e;
     } else {
       stage_begin(ST_ADMM_PRODUCT, st);
       launch_admm_product(st, K, Kp, npad, S, w.Rp, ib.rcur, w.D, ib.dcur, ib.fcur, force, tau,

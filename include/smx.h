@@ -117,6 +117,22 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
                       const double* num_per_bin, const double* v0, const double* sdisp,
                       void* workspace, double* lib, double* valid, double* T_int,
                       double* t_final, void* stream);
+/* No counterpart in the reference: smx_build_library for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * smx_build_library without the workspace: the accumulator slabs live in a
+ * fixed device array of SMX_NOALLOC_SCRATCH_BYTES that each GPU holds for the
+ * life of the process, so n * r * (P rounded up to 8 + 8 B) * 8 bytes
+ * (smx_library_workspace_bytes) must fit in it; larger calls are rejected
+ * (code 2, message in smx_last_error).  No hipMalloc / hipFree.  Calls that
+ * overlap on one GPU share that array and race. */
+#define SMX_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int smx_build_library_noalloc(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                              const double* wb, const double* T_total, const double* dt0,
+                              int64_t n, int32_t r, const int32_t* bin_mapping,
+                              const double* num_per_bin, const double* v0, const double* sdisp,
+                              double* lib, double* valid, double* T_int, double* t_final,
+                              void* stream);
 /* Trajectory variant, no symmetry, dt normalised (integrates.py:319-581):
  * o_traj (n_int, N_max, 6) with velocities in km/s, t_traj (n_int, N_max)
  * the cumulative time, and the realization average lib (n, P + 6B). */

@@ -78,6 +78,9 @@ _SIGS = [
     ('smx_build_library', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ('smx_build_library_noalloc', ctypes.c_int,
+     [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ('smx_build_library_ckpt', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P]),

@@ -425,6 +425,57 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
                        workspace, lib, valid, T_int, t_final, nullptr, 0, stream);
 }
 
+// ---------------------------------------------------------------------------
+// smx_build_library without device allocation: the per-integration
+// accumulator slabs (the workspace of smx_build_library) live in a fixed
+// file-scope device array, `scratch`, of SMX_NOALLOC_SCRATCH_BYTES.  Each GPU
+// the library runs on holds its own copy for the life of the process.  Its
+// device address is looked up once per device (hipGetSymbolAddress, with the
+// device current) and kept; the array's name is only a host-side stand-in.
+// Every call on one GPU shares that GPU's array: the slabs are cleared by a
+// fill kernel at the start of each call, and calls that overlap on one GPU
+// (other streams or threads) race on it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[SMX_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[64] = {};
+
+static int scratch_for_current_device(double** out) {
+  int dev = -1;
+  SMX_HIP(hipGetDevice(&dev));
+  SMX_REQUIRE(dev >= 0 && dev < 64, "smx_build_library_noalloc: device id %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    SMX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SMX_REQUIRE(ptr != nullptr, "smx_build_library_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+int smx_build_library_noalloc(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
+                              const double* wb, const double* T_total, const double* dt0,
+                              int64_t n, int32_t r, const int32_t* bin_mapping,
+                              const double* num_per_bin, const double* v0, const double* sdisp,
+                              double* lib, double* valid, double* T_int, double* t_final,
+                              void* stream) {
+  SMX_REQUIRE(g, "smx_build_library_noalloc: null pointer");
+  SMX_REQUIRE(n >= 0 && r >= 1, "smx_build_library_noalloc: bad sizes");
+  if (int e = check_grids(g)) return e;
+  const int64_t need = smx_library_workspace_bytes(g, n * r);
+  SMX_REQUIRE(need <= SMX_NOALLOC_SCRATCH_BYTES,
+              "smx_build_library_noalloc: %lld integrations need %lld bytes of accumulator slabs, "
+              "more than the %lld-byte scratch array; use smx_build_library with a workspace",
+              (long long)(n * r), (long long)need, (long long)SMX_NOALLOC_SCRATCH_BYTES);
+  double* ws = nullptr;
+  if (int e = scratch_for_current_device(&ws)) return e;
+  return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp, ws,
+                       lib, valid, T_int, t_final, nullptr, 0, stream);
+}
+
 int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                            const double* wb, const double* T_total, const double* dt0, int64_t n,
                            int32_t r, const int32_t* bin_mapping, const double* num_per_bin,

@@ -27,8 +27,24 @@ h_to_V_sigma               Gaussian-LOSVD moments by quadrature
 ADMM NNLS                  scipy.optimize.nnls on the stacked problem (KKT)
 theta maps                 round trip with the forward NFW map
 rotation matrix            orthogonality, det = 1, beta = 0 special case
-Jeans moments              scipy.integrate.quad on the same integrand
+Jeans moments              scipy.integrate.quad on the same integrand;
+                           v_phi on a separable model in closed form
+rotation curve, T_orb      Kepler closed forms; the R_safe floor and clamps
+clockwise cyl -> Cartesian speed, radial component, L_z = -R v_phi
+density target             tensor Gauss-Legendre cell integrals of rho R
+data normalisation,        hand-derived two-aperture example (mass per
+gamma_kin                  orbit, M/L, pixel area, GH normalisation)
+bootstrap chi^2            hand-derived rows: broadcast density term,
+                           log-mean-exp, sigma_amplify, NaN dropping
+realization average        hand values with the (sum valid + 0.1) divisor
+trajectory path            t_traj = running sum of accepted steps by the
+                           single-step function; km/s factor; dt / T
+                           normalisation sums to 1 inside the grid
+ADMM at full shape         torch FP64 restatement (tests/fp64_admm.py)
+                           equals this one on a small problem
 =========================  ===================================================
+
+Pins live in ``tests/test_oracle_*.py``.
 
 Parts with no independent pin are listed in ``UNPINNED`` below.
 """

@@ -254,8 +254,13 @@ def test_model_likelihood_gradient(galaxy):
 
 def test_gradient_batch_matches_per_model(galaxy):
     """The batched forward + adjoint returns, per theta and orbit count, the
-    single-model logL and gradient."""
+    single-model logL and gradient, and both agree with the oracle: logL to
+    1e-4 and the gradient to 1e-4 of its largest component against the
+    oracle's frozen-decision central differences (with the oracle's own NNLS
+    weights, whose difference from the GPU's is ~1e-5)."""
     from schwarmax_amd import grad as GR, synth
+    from oracle import model as OM
+    from galaxies import oracle_likelihood
     theta, dd, sched = galaxy
     rng = np.random.default_rng(5)
     thetas = np.stack([theta] + synth.batched_thetas(1, rng))
@@ -272,6 +277,14 @@ def test_gradient_batch_matches_per_model(galaxy):
                                               schedule=sched, nnls_maxiter=20, ckpt_every=16)
         assert float(v[m]) == float(v1), m
         assert np.array_equal(g[m].numpy(), g1.numpy()), m
+        frac = oracle_likelihood(thetas[m], dm, NAP, sched, 20)[3]
+        f0, fd, _ = OM.model_likelihood_grad_fd(thetas[m], dm, NAP, dd['Rzphi_n_tot'],
+                                                schedule=sched, nnls_maxiter=20)
+        err = np.abs(g[m].numpy() - fd) / np.max(np.abs(fd))
+        print(f"model {m}: valid {frac:.3f} logL rel {abs(float(v[m]) - f0) / abs(f0):.2e} "
+              f"gradient max rel {err.max():.2e}", flush=True)
+        assert abs(float(v[m]) - f0) <= 1e-4 * abs(f0), (m, float(v[m]), f0)
+        assert np.all(err < 1e-4), (m, np.stack([g[m].numpy(), fd, err]))
 
 
 def test_gradient_finite_where_jeans_moments_clamp():

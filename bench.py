@@ -39,73 +39,127 @@ def make_workload(config, seed, n=None):
     return theta, dd, sched
 
 
-def cpu_baseline(config, seed, n_int_cols=64, n_nnls=1000):
-    """Oracle (NumPy FP64) on a bounded sample of the same workload."""
+def _cpu_columns(job):
+    """Worker of the CPU baseline: the oracle's ICs, integration and binning
+    for orbit columns [c0, c0 + c) of the workload's galaxy (one thread)."""
+    config, seed, c0, c = job
     import threadpoolctl
     from schwarmax_amd import synth
-    from oracle import model as OM, integrate as OI, nnls as ON, potential as OP
+    from oracle import model as OM, integrate as OI
+    with threadpoolctl.threadpool_limits(1):
+        theta, dd, sched = synth.make_galaxy(config, seed=seed)
+        ph, pb = OM.theta_to_params(theta)
+        s = OM.schedule_of(sched)
+        sub = {k: (v[c0:c0 + c] if k in ('w0', 'jeans_normals', 'jitter_uniforms') else v)
+               for k, v in dd.items()}
+        t0 = time.time()
+        wb, Tt, dt0 = OM.initial_conditions(sub, OM.baryon_params(pb), ph, 4, s['N_step_per_orb'],
+                                            s['N_dynamical_time'])
+        rot = OM.make_rotation_matrix(pb['alpha'], pb['beta'], pb['gamma'])
+        Rl, Rn, xyl, xyn = OM.grid_args(dd)
+        og = OI.Grids(int(dd['total_bins']), dd['bin_mapping'], dd['num_per_bin'], Rl, xyl, Rn,
+                      xyn, dd['v0'], dd['s'], rot)
+        acc_fn, pot_fn = OI.make_acc_pot(OM.baryon_params(pb), ph)
+        OI.integrate_chunked(wb.reshape(-1, 6), acc_fn, pot_fn, s['N_max'], Tt.reshape(-1),
+                             dt0.reshape(-1), -pb['Omega_bar'], s['atol'], s['rtol'], s['dt_min'],
+                             s['dt_max'], og, chunk_size=s['chunk'])
+        return time.time() - t0
+
+
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(config, seed, cols_per_worker=16, n_nnls=1000, repeats=3):
+    """The oracle (NumPy FP64) timed on the host on a bounded sample of the
+    workload: a process pool over orbit columns (one process per core of
+    this process's CPU share, at most 16) for the initial conditions,
+    integration and binning, and BLAS/LAPACK on the same cores for an NNLS
+    of the workload's m rows, K right-hand sides and T iterations at
+    n_nnls columns; 1 warm-up and `repeats` timed runs of each, median.
+
+    Must run before the process touches the GPU: the pool forks it."""
+    import multiprocessing
+    import threadpoolctl
+    from schwarmax_amd import synth
+    from oracle import nnls as ON
     theta, dd, sched = synth.make_galaxy(config, seed=seed)
     cfg = synth.CONFIGS[config]
     n_full = dd['w0'].shape[0]
-    ph, pb = OM.theta_to_params(theta)
-    s = OM.schedule_of(sched)
-    threads = min(16, os.cpu_count() or 1)
-    sub = {k: (v[:n_int_cols] if k in ('w0', 'jeans_normals', 'jitter_uniforms') else v)
-           for k, v in dd.items()}
-    t0 = time.time()
-    wb, Tt, dt0 = OM.initial_conditions(sub, OM.baryon_params(pb), ph, 4, s['N_step_per_orb'],
-                                        s['N_dynamical_time'])
-    t_ics = time.time() - t0
-    rot = OM.make_rotation_matrix(pb['alpha'], pb['beta'], pb['gamma'])
-    Rl, Rn, xyl, xyn = OM.grid_args(dd)
-    og = OI.Grids(int(dd['total_bins']), dd['bin_mapping'], dd['num_per_bin'], Rl, xyl, Rn, xyn,
-                  dd['v0'], dd['s'], rot)
-    acc_fn, pot_fn = OI.make_acc_pot(OM.baryon_params(pb), ph)
-    t0 = time.time()
-    OI.integrate_chunked(wb.reshape(-1, 6), acc_fn, pot_fn, s['N_max'], Tt.reshape(-1),
-                         dt0.reshape(-1), -pb['Omega_bar'], s['atol'], s['rtol'], s['dt_min'],
-                         s['dt_max'], og, chunk_size=s['chunk'])
-    t_int = time.time() - t0
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    jobs = [(config, seed, w * cols_per_worker, cols_per_worker) for w in range(workers)]
+    cols = workers * cols_per_worker
+
+    def timed(fn):
+        fn()                                       # warm-up
+        return float(np.median([fn() for _ in range(repeats)]))
+
+    with multiprocessing.get_context('fork').Pool(workers) as pool:
+        def run_cols():
+            t0 = time.time()
+            pool.map(_cpu_columns, jobs)
+            return time.time() - t0
+        t_int = timed(run_cols)
     # NNLS sample: n_nnls columns, the workload's m rows, K rhs, T iterations.
     # Each part is scaled by its own order in n: Gram U^T U ~ n^2 m, Cholesky
     # ~ n^3, one ADMM iteration (two triangular solves per rhs) ~ n^2 K.
     rng = np.random.default_rng(1)
-    m = int(np.prod(Rn)) + 5 * int(dd['total_bins'])
+    nR, nz, nphi = (int(v) for v in dd['Rzphi_n_grid'])
+    m = nR * nz * nphi + 5 * int(dd['total_bins'])
     K, T = cfg['K'], cfg['T']
     U = rng.random((m, n_nnls)) * (rng.random((m, n_nnls)) < 0.05)
     Y = rng.random((K, m))
-    with threadpoolctl.threadpool_limits(threads):
+    parts = {}
+
+    def run_nnls():
         t0 = time.time()
         Q = U.T @ U
-        t_gram = time.time() - t0
-        t0 = time.time()
+        t1 = time.time()
         _, rho, Lc = ON.gram_factor(U, 1.0)
-        t_fac = max(time.time() - t0 - t_gram, 0.0)
-        t0 = time.time()
+        t2 = time.time()
         ON.admm(Lc, rho, -(Y @ U), 1.0, T)
-        t_admm = time.time() - t0
-    del Q
-    scale_n = n_full / n_int_cols
+        t3 = time.time()
+        parts.setdefault('gram', []).append(t1 - t0)
+        parts.setdefault('fac', []).append(max(t2 - t1 - (t1 - t0), 0.0))
+        parts.setdefault('admm', []).append(t3 - t2)
+        del Q
+        return t3 - t0
+    with threadpoolctl.threadpool_limits(workers):
+        timed(run_nnls)
+    t_gram, t_fac, t_admm = (float(np.median(parts[k][1:])) for k in ('gram', 'fac', 'admm'))
+    scale_n = n_full / cols
     r = n_full / n_nnls
-    est = (t_ics + t_int) * scale_n + t_gram * r ** 2 + t_fac * r ** 3 + t_admm * r ** 2
-    sample = (f"oracle on {n_int_cols} of {n_full} columns x4 realizations x{s['N_max']} steps "
-              f"(ICs {t_ics:.1f}s + integrate/bin {t_int:.1f}s, x{scale_n:.0f}) and NNLS at "
-              f"n={n_nnls}, m={m}, K={K}, T={T} (Gram {t_gram:.2f}s x{r:.0f}^2, Cholesky "
-              f"{t_fac:.2f}s x{r:.0f}^3, ADMM {t_admm:.1f}s x{r:.0f}^2); extrapolated "
-              f"{est:.0f} s/eval; NumPy elementwise on 1 thread, BLAS/LAPACK on {threads}")
-    return {'value': 1.0 / est, 'unit': 'model evals/s', 'cores': threads, 'kind': 'port',
-            'sample': sample}
+    est = t_int * scale_n + t_gram * r ** 2 + t_fac * r ** 3 + t_admm * r ** 2
+    sample = (f"oracle (NumPy FP64) on {cols} of {n_full} columns x4 realizations "
+              f"x{sched['N_max']} steps in {workers} processes ({t_int:.1f} s, x{scale_n:.1f}) and "
+              f"NNLS at n={n_nnls}, m={m}, K={K}, T={T} on {workers} BLAS threads (Gram "
+              f"{t_gram:.2f} s x{r:.0f}^2, Cholesky {t_fac:.2f} s x{r:.0f}^3, ADMM {t_admm:.1f} s "
+              f"x{r:.0f}^2); median of {repeats} after 1 warm-up; extrapolated {est:.0f} s/eval. "
+              f"Reference's own CPU figure: ~60-100 s/eval at 7,500 columns, float32 "
+              f"(README.md:92-93)")
+    return {'value': 1.0 / est, 'unit': 'model evals/s', 'cores': workers, 'kind': 'port',
+            'sample': sample, 'nproc': os.cpu_count(), 'cpu_model': _cpu_model(),
+            's_per_eval': est}
 
 
-def traffic_per_launch(config):
+def traffic_per_launch(config, full_frac):
     """HBM bytes per ADMM-product launch from the committed rocprofv3 --pmc
-    summary for this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
-    same counter's reading on a copy of known size), or None."""
+    summary for this workload (FETCH_SIZE + WRITE_SIZE per full and per delta
+    launch, each scaled by the same counter's reading on a copy of known
+    size), weighted by this run's share of full launches; or None."""
     path = os.path.join(ROOT, 'profiles', f'admm_gemm_kernel.{config}.traffic.json')
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get('bytes_per_launch')
+        t = json.load(f)
+    return full_frac * t['bytes_per_full_launch'] + (1.0 - full_frac) * t['bytes_per_delta_launch']
 
 
 def bench_jackknife(args):
@@ -290,6 +344,12 @@ def main():
               f"integrating a share of the orbits, with one all-reduce of the Gram)",
               file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if not args.no_cpu and not args.calibrate_copy and args.config in ('plain-20k', 'plain-2k'):
+        # before anything touches the GPU: the baseline's process pool forks
+        log('cpu baseline ...')
+        cpu = cpu_baseline(args.config, args.seed)
+        log(f"cpu baseline: {cpu['s_per_eval']:.0f} s/eval on {cpu['cores']} cores")
     import torch
     assert torch.cuda.is_available(), 'needs a GPU'
     if args.calibrate_copy:
@@ -387,7 +447,7 @@ def main():
     else:
         ach = prod_bytes / (prod_ms * 1e-3) / 1e9
         roof = {'bound': 'hbm', 'achieved': ach, 'peak': PEAK['hbm'], 'unit': 'GB/s',
-                'frac': ach / PEAK['hbm'], 'traffic': traffic_per_launch(args.config),
+                'frac': ach / PEAK['hbm'], 'traffic': traffic_per_launch(args.config, full_frac),
                 'kernel': 'admm_gemm_kernel', 'per_launch': prod_bytes,
                 'ms_per_launch': prod_ms, 'launches': stages['admm_product'][1],
                 'full_products': NN.STATS['full_products'], 'products': NN.STATS['products']}
@@ -406,9 +466,8 @@ def main():
     if grad_mode:
         out['gradient_to_forward'] = (dt / args.steps) / fwd_s
         out['config']['ckpt_every'] = 64
-    if not args.no_cpu:
-        log('cpu baseline ...')
-        out['cpu_baseline'] = cpu_baseline(args.config, args.seed)
+    if cpu is not None:
+        out['cpu_baseline'] = cpu
     print(json.dumps(out))
 
 

@@ -229,8 +229,9 @@ __global__ void split_sum_kernel(int64_t M, int64_t N, int S, const T* __restric
 }
 
 // work (optional, work_elems elements): when the output has fewer 128 x 128
-// tiles than the chip has CUs, the reduction is split over up to 8 blocks
-// of workgroups whose partials go to work and are summed in T.
+// tiles than four per CU, the reduction is split over up to 8 blocks of
+// workgroups (about four workgroups per CU, for latency hiding at one
+// output tile per workgroup) whose partials go to work and are summed in T.
 template <class T>
 static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, const T* A, int64_t lda,
                    const T* B, int64_t ldb, T beta, T* C, int64_t ldc, int lower_only = 0,
@@ -242,8 +243,8 @@ static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, con
   dim3 grid((unsigned)((N + NT_TN - 1) / NT_TN), (unsigned)((M + NT_TM - 1) / NT_TM));
   int64_t S = 1, kchunk = 0;
   const int64_t tiles = (int64_t)grid.x * grid.y;
-  if (work && !lower_only && tiles < 256) {
-    S = (256 + tiles - 1) / tiles;
+  if (work && !lower_only && tiles < 1024) {
+    S = (1024 + tiles - 1) / tiles;
     if (S > 8) S = 8;
     kchunk = ((K + S - 1) / S + NT_TK - 1) / NT_TK * NT_TK;
     S = (K + kchunk - 1) / kchunk;

@@ -330,28 +330,33 @@ __global__ void copy_out_kernel(int64_t total, const double* __restrict__ x, int
 // row maxima the previous update collected, identically in the product and
 // the update kernel, so the loop needs no host synchronisation.
 //
-// Work split: a workgroup (4 waves) owns 128 rows of R x 256 columns of M and
-// a contiguous range of the reduction index i; it writes its partial sum (no
-// atomics).  The update kernel adds the partials and applies the ADMM step.
-// Symmetry of M lets the B operand be read along rows: B[i][j] = M[j][i].
+// Work split: a workgroup (4 waves, 2 per CU) owns 128 rows of R x 256
+// columns of M and a contiguous range of the reduction index i; it writes
+// its partial sum (no atomics).  The update kernel adds the partials and
+// applies the ADMM step.  Symmetry of M lets the B operand be read along
+// rows: B[i][j] = M[j][i].
+//
+// Data paths.  M streams once per launch from HBM, straight into the waves'
+// registers: its planes are stored in the order the MFMA's B operand wants
+// them (fragment-blocked, below), so each wave loads its own 64 columns with
+// one contiguous 1 KB run per (column block, 16-index step, plane) and no
+// staging through LDS.  R is shared by every wave: each chunk (32 reduction
+// indices x 128 rows, FP32, from L2) is split into fp16 hi / lo once per
+// workgroup into a double-buffered LDS stage.  Loads run two chunks ahead of
+// the MFMAs (two register buffers of M fragments), with one barrier per chunk.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int AG_KR = 128;   // rows of R per workgroup
 constexpr int AG_BN = 256;   // columns of M per workgroup
-constexpr int AG_OCC = 2;    // workgroups per CU (LDS and registers)
+constexpr int AG_OCC = 2;    // workgroups per CU (registers: 2 waves per SIMD)
 constexpr int AG_KP = 32;    // reduction indices per chunk (npad is a multiple)
-// Waves per ADMM product workgroup: 8 waves of 128 x 32 outputs (128
-// VGPRs, four waves per SIMD at two workgroups per CU) run the product 4%
-// faster than 4 waves of 128 x 64 (240 VGPRs, two waves per SIMD): with M's
-// loads taken out the kernel still takes 80% of its time, so the chunk
-// pipeline (staging stores, barriers, fragment reads, MFMAs), not HBM, sets
-// it, and more waves hide more of its latencies.
-constexpr int AG_WAVES = 8;
-constexpr int AG_NT = 64 * AG_WAVES;            // its threads
+constexpr int AG_WAVES = 4;
+constexpr int AG_NT = 64 * AG_WAVES;            // threads of a workgroup
 constexpr int AG_WCB = AG_BN / (32 * AG_WAVES); // 32-column blocks per wave
-static_assert(AG_WCB >= 1 && AG_WCB * 32 * AG_WAVES == AG_BN, "waves tile the 256 columns");
+static_assert(AG_WCB == 2, "a wave owns 64 columns: two 32-column MFMA blocks");
+static_assert(AG_NT == 256 && AG_KR == 128, "the R chunk staging moves 4 float4 per thread");
 constexpr float AG_TAU = 1.0f / 1024.0f;   // default delta-product bound max|R - R_b| / max|R| per row
 // Anchor refresh period of the ADMM loop.  A split product's rounding (the
 // fp16 hi / lo representation and the FP32 accumulation over n / S terms of
@@ -383,169 +388,187 @@ __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
   return ldexpf(1.f, target_exp - e);
 }
 
-// Chunk-blocked layouts of the split planes, so that every (column block,
-// chunk) tile a workgroup loads is one contiguous run:
-//   M: tile (J, c) = M[256 J + r][32 c + q], r < 256, q < 32, at
-//      ((J * nch + c) * 256 + r) * 32 + q   (nch = npad / 32);
-//   R (FP32, split into fp16 hi / lo as the product loads it): tile c =
-//      R[k][32 c + q], k < Kp, at (c * Kp + k) * 32 + q.
-__device__ __forceinline__ int64_t m_tile_index(int64_t j, int64_t i, int64_t nch) {
-  return ((j / AG_BN * nch + i / AG_KP) * AG_BN + j % AG_BN) * AG_KP + i % AG_KP;
+// Fragment-blocked layout of the M planes.  The B operand of
+// v_mfma_f32_32x32x16_f16 gives lane l = 32 g + cc column cc of a 32-column
+// block and reduction indices 8 g .. 8 g + 7 of a 16-index step, so
+//   M[j][i] -> (((jb * nch + c) * 2 + t) * 64 + 32 g + cc) * 8 + e,
+// jb = j / 32, cc = j % 32, c = i / 32, t = i % 32 / 16, g = i % 16 / 8,
+// e = i % 8 (nch = npad / 32): one (column block, chunk, step) is a
+// contiguous 1 KB run in lane order.
+__device__ __forceinline__ int64_t m_frag_index(int64_t j, int64_t i, int64_t nch) {
+  const int64_t r = i % AG_KP;
+  return ((((j / 32) * nch + i / AG_KP) * 2 + r / 16) * 64 + 32 * ((r % 16) / 8) + j % 32) * 8 +
+         r % 8;
 }
 
-// M (n, n) FP64 -> scaled fp16 hi / lo (npad x npad, blocked), zero padded.
+// M (n, n) FP64 -> scaled fp16 hi / lo (npad x npad, fragment-blocked), zero
+// padded.  One thread per 8 consecutive i of a row (one 16-byte piece).
 __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t npad,
                                const float* __restrict__ rho_p, _Float16* __restrict__ Mh,
                                _Float16* __restrict__ Ml) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= npad * npad) return;
-  const int64_t j = idx / npad, i = idx - j * npad;
+  if (idx >= npad * npad / 8) return;
+  const int64_t j = idx / (npad / 8), i0 = (idx - j * (npad / 8)) * 8;
   const double sc = pow2_scale_for(1.f / *rho_p, 13);   // power of two; |M_ij| <= 1/rho
-  const double v = (i < n && j < n) ? M[j * n + i] * sc : 0.0;
-  const _Float16 h = static_cast<_Float16>(v);
-  const int64_t d = m_tile_index(j, i, npad / AG_KP);
-  Mh[d] = h;
-  Ml[d] = static_cast<_Float16>(v - static_cast<double>(h));
+  half8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t i = i0 + e;
+    const double v = (i < n && j < n) ? M[j * n + i] * sc : 0.0;
+    h[e] = static_cast<_Float16>(v);
+    l[e] = static_cast<_Float16>(v - static_cast<double>(h[e]));
+  }
+  const int64_t d = m_frag_index(j, i0, npad / AG_KP);
+  *reinterpret_cast<half8*>(Mh + d) = h;
+  *reinterpret_cast<half8*>(Ml + d) = l;
 }
 
-// One chunk (KC reduction indices) of the ADMM product's operands.  B: 256
-// rows of M (hi and lo), thread t moves 16-byte segment t%TPR of row
-// t/TPR + RPP p, TPR = KC/8 lanes covering one row piece, RPP = AG_NT/TPR
-// rows per pass; A: 128 rows of R, same pattern, split into hi and lo on the
-// way to LDS.  The hi plane of
-// the next chunk is loaded into registers before the MFMAs of the current
-// one; the lo plane and A (from L2) after them.  Prefetching both planes
-// does not fit the registers of two workgroups per CU.
-template <int KC>
-struct AdmmStage {
-  static_assert(KC == AG_KP, "the blocked split layouts use 32-wide chunks");
-  static constexpr int TPR = KC / 8;
-  static constexpr int RPP = AG_NT / TPR;
-  static constexpr int PB_ = AG_BN / RPP;   // passes for B
-  static constexpr int PA_ = AG_KR / RPP;   // passes for A
-  uint4 bh[PB_], bl[PB_];
-  // tile (j0 / 256, i0 / 32) of the blocked M planes: thread t, pass p moves
-  // row t/4 + RPP p, segment t%4, i.e. the tile's uint4 number t + AG_NT p.
-  // M streams once per launch: non-temporal loads keep it from evicting the
-  // R tiles that the workgroups of one split share in L2.
-  __device__ static void load_plane(const _Float16* __restrict__ M, int64_t npad, int64_t j0,
-                                    int64_t i0, uint4 (&dst)[PB_]) {
-    const int t = threadIdx.x;
-    const int64_t base = ((j0 / AG_BN) * (npad / AG_KP) + i0 / AG_KP) * (int64_t)(AG_BN * AG_KP);
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// R operand staging.  Blocked FP32 layout of R (written by the update
+// kernel): chunk c = R[k][32 c + q], k < Kp, at (c * Kp + k) * 32 + q, so the
+// 128 rows of a chunk are 16 KB in a row.  Thread t moves float4 number
+// 256 u + t of it (u < 4: one contiguous 1 KB run per wave and load), i.e.
+// row 32 u + t / 8, indices 4 (t % 8) .. +3, and splits it into fp16 hi / lo
+// with that row's power-of-two scale: max |R_k| -> 2^14, so hi + lo = R *
+// scale to 2^-22 relative.  Rows k >= K (zero padding) read row K - 1
+// instead, one cache line shared by the wave, and stage zeros (scale 0).
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+struct AdmmA {
+  float4 v[4];
+  float sc[4];
+  int off[4];   // float4 offsets of this thread's pieces in a chunk
+  __device__ void scale(const unsigned* __restrict__ rowmax, int64_t K, int64_t k0) {
 #pragma unroll
-    for (int p = 0; p < PB_; ++p) {
-      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
-      const u32x4 h = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(M + off));
-      dst[p] = make_uint4(h.x, h.y, h.z, h.w);
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = k0 + 32 * u + threadIdx.x / 8;
+      sc[u] = k < K ? pow2_scale_for(__uint_as_float(rowmax[k]), 14) : 0.f;
+      off[u] = k < K ? 256 * u + threadIdx.x : (int)(K - 1 - k0) * 8 + threadIdx.x % 8;
     }
   }
-  // this thread's rows of R in the A tile (t/TPR + RPP p) and their scales:
-  // max |R_k| -> 2^14, a power of two, so hi + lo = R * scale exactly to
-  // 2^-22 relative
-  float rsc[PA_];
-  __device__ void row_scales(const unsigned* __restrict__ rowmax, int64_t k0) {
+  __device__ void load(const float* __restrict__ A, int64_t Kp, int64_t k0, int64_t c) {
+    const float4* p = reinterpret_cast<const float4*>(A + (c * Kp + k0) * AG_KP);
 #pragma unroll
-    for (int p = 0; p < PA_; ++p)
-      rsc[p] = pow2_scale_for(__uint_as_float(rowmax[k0 + threadIdx.x / TPR + RPP * p]), 14);
+    for (int u = 0; u < 4; ++u) v[u] = p[off[u]];
   }
-  // A operand chunk (this thread's PA_ rows, 8 FP32 values each) in registers
-  float4 av[PA_][2];
-  __device__ void load_a(const float* __restrict__ R, int64_t Kp, int64_t k0, int64_t i0) {
-    const int t = threadIdx.x;
-    const int64_t base = ((i0 / AG_KP) * Kp + k0) * AG_KP;
-#pragma unroll
-    for (int p = 0; p < PA_; ++p) {
-      const int64_t off = base + (int64_t)(t / TPR + RPP * p) * AG_KP + 8 * (t % TPR);
-      av[p][0] = *reinterpret_cast<const float4*>(R + off);
-      av[p][1] = *reinterpret_cast<const float4*>(R + off + 4);
-    }
-  }
-  // B planes and the A chunk from registers to LDS, A split into fp16 hi / lo
   template <bool FULL, int P>
-  __device__ void store(_Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P],
-                        _Float16 (*Al)[P]) const {
+  __device__ void store(_Float16 (*Ah)[P], _Float16 (*Al)[P]) const {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int p = 0; p < PB_; ++p) {
-      *reinterpret_cast<uint4*>(&Bh[t / TPR + RPP * p][8 * (t % TPR)]) = bh[p];
-      if (FULL) *reinterpret_cast<uint4*>(&Bl[t / TPR + RPP * p][8 * (t % TPR)]) = bl[p];
-    }
+    for (int u = 0; u < 4; ++u) {
+      const float f[4] = {v[u].x * sc[u], v[u].y * sc[u], v[u].z * sc[u], v[u].w * sc[u]};
+      half4 h, l;
 #pragma unroll
-    for (int p = 0; p < PA_; ++p) {
-      const float v[8] = {av[p][0].x, av[p][0].y, av[p][0].z, av[p][0].w,
-                          av[p][1].x, av[p][1].y, av[p][1].z, av[p][1].w};
-      half8 h, l;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const float sv = v[e] * rsc[p];
-        h[e] = static_cast<_Float16>(sv);
-        if (FULL) l[e] = static_cast<_Float16>(sv - static_cast<float>(h[e]));
+      for (int e = 0; e < 4; ++e) {
+        h[e] = static_cast<_Float16>(f[e]);
+        if (FULL) l[e] = static_cast<_Float16>(f[e] - static_cast<float>(h[e]));
       }
-      *reinterpret_cast<half8*>(&Ah[t / TPR + RPP * p][8 * (t % TPR)]) = h;
-      if (FULL) *reinterpret_cast<half8*>(&Al[t / TPR + RPP * p][8 * (t % TPR)]) = l;
+      *reinterpret_cast<half4*>(&Ah[32 * u + t / 8][4 * (t % 8)]) = h;
+      if (FULL) *reinterpret_cast<half4*>(&Al[32 * u + t / 8][4 * (t % 8)]) = l;
     }
   }
 };
 
-// Main loop over the split's chunks.  FULL: hi*hi + hi*lo + lo*hi of R and
-// M; otherwise hi*hi of D = R - R_b and M's hi plane.
-template <int KC, bool FULL, int P>
-__device__ __forceinline__ void admm_gemm_loop(int64_t Kp, int64_t npad, int64_t k0, int64_t j0,
-                                               int64_t ch0, int64_t ch1, const float* __restrict__ A,
-                                               const _Float16* __restrict__ Mh,
-                                               const _Float16* __restrict__ Ml, AdmmStage<KC>& stg,
-                                               _Float16 (*Bh)[P], _Float16 (*Bl)[P], _Float16 (*Ah)[P],
-                                               _Float16 (*Al)[P], f32x16 (&acc)[4][AG_WCB]) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = lane >> 5, c = lane & 31;
-  if (ch0 < ch1) {
-    stg.load_plane(Mh, npad, j0, ch0 * KC, stg.bh);
-    if (FULL) stg.load_plane(Ml, npad, j0, ch0 * KC, stg.bl);
-    stg.load_a(A, Kp, k0, ch0 * KC);
-    stg.template store<FULL>(Bh, Bl, Ah, Al);
-  }
-  __syncthreads();
-  for (int64_t ch = ch0; ch < ch1; ++ch) {
-    const bool more = ch + 1 < ch1;
-    if (more) {
-      stg.load_plane(Mh, npad, j0, (ch + 1) * KC, stg.bh);
-      if (!FULL) stg.load_a(A, Kp, k0, (ch + 1) * KC);
-    }
+// This wave's M fragments of one chunk: hi (and, for a full product, lo)
+// planes of its AG_WCB column blocks, both 16-index steps.  M streams once
+// per launch: non-temporal loads keep it from evicting the R chunks that the
+// workgroups of one split share in L2.
+struct AdmmB {
+  half8 h[AG_WCB][2], l[AG_WCB][2];
+  template <bool FULL>
+  __device__ void load(const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml,
+                       int64_t jb0, int64_t nch, int64_t c) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int t = 0; t < KC / 16; ++t) {
-      half8 bh[AG_WCB], bl[AG_WCB];
+    for (int cb = 0; cb < AG_WCB; ++cb)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int64_t off = ((((jb0 + cb) * nch + c) * 2 + t) * 64 + lane) * 8;
+        h[cb][t] = __builtin_nontemporal_load(reinterpret_cast<const half8*>(Mh + off));
+        if (FULL) l[cb][t] = __builtin_nontemporal_load(reinterpret_cast<const half8*>(Ml + off));
+      }
+  }
+};
+
+// MFMAs of one chunk: this wave's 128 x 64 outputs (4 row blocks x AG_WCB
+// column blocks of 32 x 32).  FULL: hi*hi + hi*lo + lo*hi of R and M;
+// otherwise hi*hi of D = R - R_b and M's hi plane.
+template <bool FULL, int P>
+__device__ __forceinline__ void admm_mfma_chunk(const _Float16 (*Ah)[P], const _Float16 (*Al)[P],
+                                                const AdmmB& b, f32x16 (&acc)[4][AG_WCB]) {
+  const int lane = threadIdx.x & 63, g = lane >> 5, c = lane & 31;
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
+      half8 al;
+      if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
 #pragma unroll
       for (int cb = 0; cb < AG_WCB; ++cb) {
-        bh[cb] = *reinterpret_cast<const half8*>(&Bh[wave * (AG_BN / AG_WAVES) + cb * 32 + c][16 * t + 8 * g]);
-        if (FULL)
-          bl[cb] = *reinterpret_cast<const half8*>(&Bl[wave * (AG_BN / AG_WAVES) + cb * 32 + c][16 * t + 8 * g]);
-      }
-#pragma unroll
-      for (int rb = 0; rb < 4; ++rb) {
-        const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
-        half8 al;
-        if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
-#pragma unroll
-        for (int cb = 0; cb < AG_WCB; ++cb) {
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[cb], acc[rb][cb], 0, 0, 0);
-          if (FULL) {
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[cb], acc[rb][cb], 0, 0, 0);
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[cb], acc[rb][cb], 0, 0, 0);
-          }
+        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b.h[cb][t], acc[rb][cb], 0, 0, 0);
+        if (FULL) {
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b.l[cb][t], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b.h[cb][t], acc[rb][cb], 0, 0, 0);
         }
       }
     }
-    __syncthreads();
-    if (more) {
-      if (FULL) {
-        stg.load_plane(Ml, npad, j0, (ch + 1) * KC, stg.bl);
-        stg.load_a(A, Kp, k0, (ch + 1) * KC);
-      }
-      stg.template store<FULL>(Bh, Bl, Ah, Al);
-    }
-    __syncthreads();
+}
+
+// One step of the chunk loop: MFMAs of the chunk in stage s (M fragments in
+// b); stage the next chunk's R piece (loaded one step earlier into a) into
+// the other stage; load the R piece and M fragments of chunk c2 (two steps
+// ahead) into a and b; one barrier.  The R piece is loaded before the M
+// fragments, so waiting for it never waits for M loads issued later: M
+// fragments stay in flight for two steps.
+template <bool FULL, int P>
+__device__ __forceinline__ void admm_step(int s, int64_t c2, const float* __restrict__ A, int64_t Kp,
+                                          int64_t k0, const _Float16* __restrict__ Mh,
+                                          const _Float16* __restrict__ Ml, int64_t jb0, int64_t nch,
+                                          _Float16 (*Ah)[AG_KR][P], _Float16 (*Al)[AG_KR][P],
+                                          AdmmA& a, AdmmB& b, f32x16 (&acc)[4][AG_WCB]) {
+  admm_mfma_chunk<FULL, P>(Ah[s], Al[s], b, acc);
+  a.template store<FULL, P>(Ah[s ^ 1], Al[s ^ 1]);
+  a.load(A, Kp, k0, c2);
+  __builtin_amdgcn_sched_barrier(0);   // keep the R loads ahead of the M loads
+  b.template load<FULL>(Mh, Ml, jb0, nch, c2);
+  __syncthreads();
+}
+
+// Main loop over the split's chunks [ch0, ch1): pairs of steps (stage 0 /
+// fragments b0, stage 1 / fragments b1), then at most one chunk.  M
+// fragments are loaded two steps ahead, R pieces one step ahead.  Every step
+// issues its loads, for the last chunk again once the range is exhausted (in
+// bounds, unused): a loop without conditional loads lets the compiler count
+// outstanding loads exactly, so waiting for one chunk's data does not wait
+// for a later chunk's.  The prologue issues in the loop's order, so the loop
+// entry sees the outstanding loads of the steady state.
+template <bool FULL, int P>
+__device__ __forceinline__ void admm_gemm_loop(int64_t K, int64_t Kp, int64_t nch, int64_t k0, int64_t jb0,
+                                               int64_t ch0, int64_t ch1, const float* __restrict__ A,
+                                               const unsigned* __restrict__ rowmax,
+                                               const _Float16* __restrict__ Mh,
+                                               const _Float16* __restrict__ Ml,
+                                               _Float16 (*Ah)[AG_KR][P], _Float16 (*Al)[AG_KR][P],
+                                               f32x16 (&acc)[4][AG_WCB]) {
+  const int64_t nq = ch1 - ch0;
+  if (nq <= 0) return;   // block-uniform
+  const auto chunk = [&](int64_t q) { return ch0 + (q < nq ? q : nq - 1); };
+  AdmmA a;
+  AdmmB b0, b1;
+  a.scale(rowmax, K, k0);
+  a.load(A, Kp, k0, ch0);
+  __builtin_amdgcn_sched_barrier(0);
+  b0.template load<FULL>(Mh, Ml, jb0, nch, ch0);
+  a.template store<FULL, P>(Ah[0], Al[0]);
+  a.load(A, Kp, k0, chunk(1));
+  __builtin_amdgcn_sched_barrier(0);
+  b1.template load<FULL>(Mh, Ml, jb0, nch, chunk(1));
+  __syncthreads();
+  int64_t q = 0;
+  for (; q + 2 <= nq; q += 2) {
+    admm_step<FULL, P>(0, chunk(q + 2), A, Kp, k0, Mh, Ml, jb0, nch, Ah, Al, a, b0, acc);
+    admm_step<FULL, P>(1, chunk(q + 3), A, Kp, k0, Mh, Ml, jb0, nch, Ah, Al, a, b1, acc);
   }
+  if (q < nq) admm_mfma_chunk<FULL, P>(Ah[0], Al[0], b0, acc);   // odd count: the last chunk
 }
 
 // Bijection of [0, G): block b, dispatched to XCD b % 8, gets the index that
@@ -565,11 +588,10 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
     const unsigned* __restrict__ rmax, const float* __restrict__ D,
     const unsigned* __restrict__ dmax, const unsigned* __restrict__ fmax_, int force, float tau,
     const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml, float* __restrict__ part) {
-  constexpr int P = KC + 8;
-  __shared__ __align__(16) _Float16 Bh[AG_BN][P];
-  __shared__ __align__(16) _Float16 Bl[AG_BN][P];
-  __shared__ __align__(16) _Float16 Ah[AG_KR][P];
-  __shared__ __align__(16) _Float16 Al[AG_KR][P];
+  static_assert(KC == AG_KP, "the blocked layouts use 32-index chunks");
+  constexpr int P = KC + 8;   // LDS row pitch (halfs): conflict-free fragment reads
+  __shared__ __align__(16) _Float16 Ah[2][AG_KR][P];
+  __shared__ __align__(16) _Float16 Al[2][AG_KR][P];
   const bool full = admm_full_product(force, K, fmax_, dmax, tau);
   // 1-D grid, row block of R fastest (the Kp/128 row blocks that share a tile
   // of M run back to back and share it through the caches), then column
@@ -582,11 +604,12 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
   const int64_t k0 = (bid % nrb) * AG_KR;
   const int64_t j0 = ((bid / nrb) % ncb) * AG_BN;
   const int split = (int)(bid / (nrb * ncb));
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 5, c = lane & 31;
   const int64_t nch = npad / KC;
   const int64_t ch0 = (int64_t)split * chunks_per_split;
   const int64_t ch1 = ch0 + chunks_per_split < nch ? ch0 + chunks_per_split : nch;
+  const int64_t jb0 = j0 / 32 + AG_WCB * wave;   // this wave's first 32-column block
   f32x16 acc[4][AG_WCB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -594,18 +617,14 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
     for (int b = 0; b < AG_WCB; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  AdmmStage<KC> stg;
-  if (full) {
-    stg.row_scales(rmax, k0);
-    admm_gemm_loop<KC, true>(Kp, npad, k0, j0, ch0, ch1, R, Mh, Ml, stg, Bh, Bl, Ah, Al, acc);
-  } else {
-    stg.row_scales(dmax, k0);
-    admm_gemm_loop<KC, false>(Kp, npad, k0, j0, ch0, ch1, D, Mh, Ml, stg, Bh, Bl, Ah, Al, acc);
-  }
+  if (full)
+    admm_gemm_loop<true, P>(K, Kp, nch, k0, jb0, ch0, ch1, R, rmax, Mh, Ml, Ah, Al, acc);
+  else
+    admm_gemm_loop<false, P>(K, Kp, nch, k0, jb0, ch0, ch1, D, dmax, Mh, Ml, Ah, Al, acc);
   // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4,
   // column c.  Rows K..Kp-1 (zero padding of R) are not stored: every reader
   // of the partials sums rows k < K only.
-  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (AG_BN / AG_WAVES);
+  float* out = part + ((int64_t)split * Kp + k0) * npad + j0 + wave * (32 * AG_WCB);
   const int64_t rows = K - k0;
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
@@ -1521,7 +1540,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   const int64_t tot = K * n;
   const int64_t npad = w.npad, Kp = w.Kp;
   const int S = w.S;
-  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad, 256)), dim3(256), 0, st, w.Q64, n,
+  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad / 8, 256)), dim3(256), 0, st, w.Q64, n,
                      npad, w.rho, w.Mh, w.Ml);
   SMX_LAUNCHED();
   for (float* buf : {w.Rp, w.Rb, w.D})
@@ -1660,7 +1679,7 @@ static int nnls_build_operator(hipStream_t st, const NnlsWs& w, const double* li
   stage_end(ST_GRAM, st, 3);
   stage_begin(ST_FACTOR, st);
   if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
-  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(w.npad * w.npad, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(w.npad * w.npad / 8, 256)), dim3(256), 0, st,
                      w.Q64, n, w.npad, w.rho, w.Mh, w.Ml);
   SMX_LAUNCHED();
   stage_end(ST_FACTOR, st, 1);

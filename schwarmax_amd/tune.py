@@ -2,7 +2,7 @@
 
 The ADMM product (one launch per iteration, the largest share of a model
 evaluation) splits its reduction so that the grid holds a chosen number of
-256-thread workgroups per CU (and at least one workgroup per CU).  ``autotune_admm`` times the product kernel at each
+product workgroups per CU (and at least one workgroup per CU).  ``autotune_admm`` times the product kernel at each
 candidate setting on a synthetic problem of the workload's shape, full and
 delta launches weighted by the share of full products a solve takes, and
 keeps the fastest setting in the library (``smx_admm_set_occupancy``).  The
@@ -18,9 +18,10 @@ import torch
 
 from . import _lib
 
-# Share of full products in a 1,000-iteration solve at the plain-20k shapes
-# (65 of 1,000); the rest are delta products.
-FULL_SHARE = 0.065
+# Share of full products among the split products of a 1,000-iteration solve
+# at the plain-20k shapes (675 of 990; the other 10 iterations take FP64
+# anchors); the rest are delta products.
+FULL_SHARE = 0.68
 CANDIDATES = (1, 2, 3, 4, 5, 6, 7, 8)
 
 _RESULTS = {}

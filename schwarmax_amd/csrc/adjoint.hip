@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void tdot_kernel(const double* __restrict__ ws
         const int64_t q = e - P;
         const int64_t blk = q / B, a = q - blk * B;
         const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
-        v = ap[8 * a + comp];
+        v = ap[kApRec * a + comp];
         if (blk == 0) v = v / (npb[a] * area + kEps);
       }
       dot += gr[e] * v;

@@ -74,6 +74,7 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
     double* __restrict__ pool, unsigned* __restrict__ counter) {
   __shared__ double s_valid[64];
   __shared__ double s_T[64];
+  __shared__ double ap_lds[24 * 64];   // aperture-record cache, 4 copies x 6 terms
   __shared__ int s_item;
   const int lane = threadIdx.x;
   const smx_grids& g0 = grids[0];
@@ -106,8 +107,10 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
       const double EJ0 = jacobi(p, s.Omega, y);
       double t = 0.0, dt = M.dt0[q];
       const double T_acc =
-          g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1)
-                     : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1);
+          g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1,
+                                            ap_lds + lane)
+                     : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1,
+                                            ap_lds + lane);
       const double EJ = jacobi(p, s.Omega, y);
       valid = ((fabs(EJ / EJ0 - 1.0) < 0.1) && (t > M.T_total[q] / 10.0)) ? 1.0 : 0.0;
       T = T_acc + kEps;
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
             const int64_t q = e - P;
             const int64_t blk = q / B, a = q - blk * B;
             const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
-            v = sap[8 * a + comp] / s_T[c * r + kk];
-            sap[8 * a + comp] = 0.0;
+            v = sap[kApRec * a + comp] / s_T[c * r + kk];
+            sap[kApRec * a + comp] = 0.0;
             if (blk == 0) v = v / (npb[a] * g.area_pixel + kEps);
           }
           acc += (s_valid[c * r + kk] > 0.5 ? v : 0.0) * wt;

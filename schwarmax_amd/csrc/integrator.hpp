@@ -117,10 +117,11 @@ __device__ __forceinline__ double bs3_attempt_stages(const smx_potential& p, con
 }
 
 // per-integration accumulator slab: P cells (padded to 8) then B records of
-// 8 doubles [dt, g dt H0, ..., g dt H4, -, -]
+// kApRec = 6 doubles [dt, g dt H0, ..., g dt H4] (48 bytes, 16-byte aligned)
+constexpr int kApRec = 6;
 __host__ __device__ __forceinline__ int64_t slab_rz_len(int P) { return (P + 7) & ~7; }
 __host__ __device__ __forceinline__ int64_t slab_stride(int P, int B) {
-  return slab_rz_len(P) + 8 * (int64_t)B;
+  return slab_rz_len(P) + kApRec * (int64_t)B;
 }
 
 // Register cache of the (R, z, phi) cell each symmetry copy last hit.  An
@@ -161,14 +162,60 @@ struct CellCache {
   }
 };
 
+// Cache of the aperture record each symmetry copy last hit, the same scheme
+// as CellCache for the six record terms [dt, g dt H0..H4]: a copy's terms
+// are summed while its samples fall in one aperture and added to the slab's
+// record when it moves to another aperture, and at the end of the
+// integration.  A copy in no aperture (id -1) leaves its cached record as it
+// is.  The ids live in registers, the sums in LDS (the integrator's registers
+// are full): lane l's term k of copy c at sum[(6 c + k) * 64 + l], so a
+// wave's accesses are consecutive doubles.
+template <int NC>
+struct ApCache {
+  int id[NC];
+  double* sum;   // this lane's column of a __shared__ (6 NC) x 64 array
+  __device__ __forceinline__ void init(double* lds_col) {
+    sum = lds_col;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) id[c] = -1;
+  }
+  __device__ __forceinline__ void write(double* __restrict__ ap, int c) const {
+    if (id[c] < 0) return;
+    double* rec = ap + kApRec * (int64_t)id[c];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec[k] += sum[(6 * c + k) * 64];
+  }
+  __device__ __forceinline__ void add(double* __restrict__ ap, int c, int a, double w, double gd,
+                                      const double (&H)[5]) {
+    if (a != id[c]) {
+      write(ap, c);
+      id[c] = a;
+      sum[(6 * c) * 64] = w;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sum[(6 * c + 1 + k) * 64] = gd * H[k];
+      return;
+    }
+    sum[(6 * c) * 64] += w;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sum[(6 * c + 1 + k) * 64] += gd * H[k];
+  }
+  __device__ __forceinline__ void flush(double* __restrict__ ap) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      write(ap, c);
+      id[c] = -1;
+    }
+  }
+};
+
 // Binning of one accepted sample's NC symmetry copies with weight w: cells
-// through the register cache, aperture records read-modify-written.
+// and aperture records through their register caches.
 template <int NC>
 __device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __restrict__ bm,
                                            const double* __restrict__ v0,
                                            const double* __restrict__ sd, const double* y,
-                                           double w, CellCache<NC>& cache, double* __restrict__ rz,
-                                           double* __restrict__ ap) {
+                                           double w, CellCache<NC>& cache, ApCache<NC>& acache,
+                                           double* __restrict__ rz, double* __restrict__ ap) {
   SampleProj sp;
   project(g, y, NC > 2, sp);
 #pragma unroll
@@ -180,11 +227,7 @@ __device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __
     if (a >= 0) {
       double gw, H[5];
       gh_terms(vlos, v0[a], sd[a], gw, H);
-      double* rec = ap + 8 * (int64_t)a;
-      const double gd = gw * w;
-      rec[0] += w;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) rec[1 + k] += gd * H[k];
+      acache.add(ap, c, a, w, gw * w, H);
     }
   }
 }
@@ -193,7 +236,9 @@ __device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __
 // (integrates.py:181-205), every accepted state binned with weight dt in its
 // NC copies; returns the accumulated time (NC x the integrated time).  With
 // ckpt, the state (y, dt, t) before every ckpt_every-th attempt is stored at
-// ckpt + (attempt / ckpt_every) * ckpt_stride (8 doubles).
+// ckpt + (attempt / ckpt_every) * ckpt_stride (8 doubles).  ap_lds: this
+// lane's column of a __shared__ array of at least 6 NC x 64 doubles (the
+// aperture-record cache; the block is one wave of 64 lanes).
 template <int NC>
 __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, const smx_grids& g,
                                                     const smx_schedule& s, const int32_t* bm,
@@ -201,9 +246,12 @@ __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, cons
                                                     double* y, double* k1, double& dt,
                                                     double* __restrict__ rz,
                                                     double* __restrict__ ap, double* ckpt,
-                                                    int64_t ckpt_stride, int ckpt_every) {
+                                                    int64_t ckpt_stride, int ckpt_every,
+                                                    double* ap_lds) {
   CellCache<NC> cache;
   cache.init();
+  ApCache<NC> acache;
+  acache.init(ap_lds);
   double T_acc = 0.0;
   for (int step = 0; step < s.N_max; ++step) {
     if (ckpt != nullptr && step % ckpt_every == 0) {
@@ -215,11 +263,12 @@ __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, cons
     }
     const double used = bs3_attempt(p, s, t, y, k1, dt);
     if (used != 0.0) {
-      bin_sample<NC>(g, bm, v0, sd, y, used, cache, rz, ap);
+      bin_sample<NC>(g, bm, v0, sd, y, used, cache, acache, rz, ap);
       T_acc += NC * used;
     }
   }
   cache.flush(rz);
+  acache.flush(ap);
   return T_acc;
 }
 

@@ -86,49 +86,245 @@ __global__ __launch_bounds__(256, SMX_IC_WAVES) void initial_conditions_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Fused BS3 integration + binning (integrates.py:37-206): one integration per
-// lane, samples accumulated into the lane's own slab.  With ckpt, the state
-// (y, dt, t) before every ckpt_every-th attempt is stored for the adjoint.
+// Fused BS3 integration + binning (integrates.py:37-206), samples accumulated
+// into each integration's own slab.  With ckpt, the state (y, dt, t) before
+// every ckpt_every-th attempt is stored for the adjoint.
+//
+// Balanced over the resident waves.  The kernel runs W waves of 64 lanes,
+// W = the waves resident at once (one per SIMD at the integrator's register
+// count), and every integration makes the same L = N_max attempts.  With
+// more integrations than lanes a second round of whole integrations would
+// leave most SIMDs idle (80,000 integrations on 65,536 lanes: 1.22 rounds of
+// work in 2 rounds of time).  Instead wave w takes the contiguous share of m
+// integrations [w n / W, (w + 1) n / W) and spreads their m L attempts over
+// its lanes by McNaughton's wrap-around rule: lane l covers attempts
+// [l C, (l + 1) C) of the share's concatenation, C = max(L, ceil(m L / 64)).
+// An integration cut at a lane boundary runs its first part (its head) on
+// lane l + 1 at the start of the loop and its second part (its tail) on lane
+// l at the end of the loop, ending at step C, so the two parts never overlap
+// (C >= L); the state in between passes through LDS (the workgroup is the
+// one wave, which keeps its LDS operations in order).  Each lane runs, in
+// loop steps: its head [0, hb), its whole integrations from hb on, its tail
+// [C - (L - ta), C).  Every integration's attempts, binning and checkpoints
+// are those of integrate_and_bin, in the same order, into its own slab; the
+// register caches are flushed at a head / tail boundary, which groups the
+// slab additions differently (FP64 reassociation only).
 // ---------------------------------------------------------------------------
 #ifndef SMX_INTEGRATOR_WAVES
 #define SMX_INTEGRATOR_WAVES 1   // waves per SIMD the integrator's registers are sized for
 #endif
+
+struct IntegState {
+  double y[6], k1[6];
+  double t, dt, T_acc, EJ0;
+  int64_t job;       // integration index, or -1
+  int64_t attempt;   // attempts made so far
+};
+constexpr int kHandoff = 16;   // doubles per handed-over state
+
+// One lane's schedule: segments in loop-step order.
+struct LaneSchedule {
+  int64_t C, L;
+  int64_t jb, hb;          // head: integration jb, attempts [0, hb)
+  int64_t jf, nfull;       // whole integrations jf .. jf + nfull - 1 from step hb
+  int64_t ja, ta;          // tail: integration ja, attempts [ta, L), steps [C - L + ta, C)
+  // segment s (0 .. nseg-1) -> (kind, integration, first attempt, start step, end step)
+  __device__ void segment(int sidx, int& kind, int64_t& job, int64_t& first, int64_t& t0,
+                          int64_t& t1) const {
+    int q = sidx;
+    if (hb > 0) {
+      if (q == 0) { kind = 0; job = jb; first = 0; t0 = 0; t1 = hb; return; }
+      --q;
+    }
+    if (q < nfull) { kind = 1; job = jf + q; first = 0; t0 = hb + q * L; t1 = t0 + L; return; }
+    q -= (int)nfull;
+    kind = 2; job = ja; first = ta; t0 = C - L + ta; t1 = C;   // q == 0 and ta > 0
+  }
+  __device__ int count() const { return (hb > 0 ? 1 : 0) + (int)nfull + (ta > 0 ? 1 : 0); }
+};
+
+template <int NC>
+__device__ __forceinline__ void integrate_balanced(
+    const smx_potential& p, const smx_grids& g, const smx_schedule& s,
+    const double* __restrict__ wb, const double* __restrict__ T_total,
+    const double* __restrict__ dt0, int64_t n_int, int64_t W, const int32_t* __restrict__ bm,
+    const double* __restrict__ v0, const double* __restrict__ sd, double* __restrict__ ws,
+    double* __restrict__ valid_out, double* __restrict__ T_out, double* __restrict__ tfin_out,
+    double* __restrict__ ckpt, int ckpt_every, double* ap_col, double* handoff) {
+  const int lane = threadIdx.x;
+  const int64_t w = blockIdx.x;
+  const int64_t j0 = w * n_int / W, m = (w + 1) * n_int / W - j0;
+  const int64_t L = s.N_max;
+  if (m <= 0 || L <= 0) {
+    // no attempts: every integration of the share is its initial state
+    for (int64_t j = lane; j < m; j += 64) {
+      const int64_t i = j0 + j;
+      valid_out[i] = (0.0 > T_total[i] / 10.0) ? 1.0 : 0.0;
+      T_out[i] = kEps;
+      tfin_out[i] = 0.0;
+    }
+    return;
+  }
+  LaneSchedule ls;
+  ls.L = L;
+  ls.C = (m * L + 63) / 64 > L ? (m * L + 63) / 64 : L;
+  const int64_t a0 = lane * ls.C < m * L ? lane * ls.C : m * L;
+  const int64_t a1 = (lane + 1) * ls.C < m * L ? (lane + 1) * ls.C : m * L;
+  ls.jb = a1 / L;
+  ls.hb = ls.jb < m ? a1 - ls.jb * L : 0;
+  ls.jf = (a0 + L - 1) / L;
+  ls.nfull = a1 / L - ls.jf > 0 ? a1 / L - ls.jf : 0;
+  ls.ja = a0 / L;
+  ls.ta = ls.ja < m ? a0 - ls.ja * L : 0;
+  const int nseg = a1 > a0 ? ls.count() : 0;
+
+  const int P = g.nR * g.nz * g.nphi;
+  const int64_t stride = slab_stride(P, g.num_vbin);
+  const int64_t ckpt_stride = n_int * 8;
+  IntegState st;
+  st.job = -1;
+  st.attempt = 0;
+  CellCache<NC> cache;
+  ApCache<NC> acache;
+  cache.init();
+  acache.init(ap_col);
+  double* rz = nullptr;
+  double* ap = nullptr;
+  int sidx = 0, kind = 0;
+  int64_t job = 0, first = 0, t0 = 0, t1 = 0;
+  if (nseg > 0) ls.segment(0, kind, job, first, t0, t1);
+
+  for (int64_t step = 0; step <= ls.C; ++step) {
+    // segment ends (a head hands its state over; a whole integration or a
+    // tail is finished) for every lane, then starts, so that a tail that
+    // starts at the step its head ends reads the handed-over state
+    const bool ends = sidx < nseg && st.job >= 0 && step == t1;
+    const bool starts_now = sidx < nseg && st.job < 0 && step == t0;
+    if (__ballot(ends || starts_now) != 0) {   // wave-uniform
+      bool start = starts_now;
+      if (ends) {
+        cache.flush(rz);
+        acache.flush(ap);
+        const int64_t i = j0 + st.job;
+        if (kind == 0) {
+          double* h = handoff + lane;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            h[q * 64] = st.y[q];
+            h[(6 + q) * 64] = st.k1[q];
+          }
+          h[12 * 64] = st.t;
+          h[13 * 64] = st.dt;
+          h[14 * 64] = st.T_acc;
+          h[15 * 64] = st.EJ0;
+        } else {
+          const double EJ = jacobi(p, s.Omega, st.y);
+          const bool ok = (fabs(EJ / st.EJ0 - 1.0) < 0.1) && (st.t > T_total[i] / 10.0);
+          valid_out[i] = ok ? 1.0 : 0.0;
+          T_out[i] = st.T_acc + kEps;
+          tfin_out[i] = st.t;
+        }
+        st.job = -1;
+        ++sidx;
+        if (sidx < nseg) {
+          ls.segment(sidx, kind, job, first, t0, t1);
+          start = step == t0;
+        }
+      }
+      // handed-over states written above are read below by the next lane
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      if (start) {
+        const int64_t i = j0 + job;
+        if (kind == 2) {
+          const double* h = handoff + (lane - 1);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            st.y[q] = h[q * 64];
+            st.k1[q] = h[(6 + q) * 64];
+          }
+          st.t = h[12 * 64];
+          st.dt = h[13 * 64];
+          st.T_acc = h[14 * 64];
+          st.EJ0 = h[15 * 64];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 6; ++q) st.y[q] = wb[6 * i + q];
+          deriv(p, s.Omega, st.y, st.k1);
+          st.EJ0 = jacobi(p, s.Omega, st.y);
+          st.t = 0.0;
+          st.dt = dt0[i];
+          st.T_acc = 0.0;
+        }
+        st.job = job;
+        st.attempt = first;
+        rz = ws + i * stride;
+        ap = rz + slab_rz_len(P);
+      }
+    }
+    if (st.job >= 0 && step < ls.C) {
+      const int64_t i = j0 + st.job;
+      if (ckpt != nullptr && st.attempt % ckpt_every == 0) {
+        double* c = ckpt + (st.attempt / ckpt_every) * ckpt_stride + i * 8;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) c[q] = st.y[q];
+        c[6] = st.dt;
+        c[7] = st.t;
+      }
+      const double used = bs3_attempt(p, s, st.t, st.y, st.k1, st.dt);
+      if (used != 0.0) {
+        bin_sample<NC>(g, bm, v0, sd, st.y, used, cache, acache, rz, ap);
+        st.T_acc += NC * used;
+      }
+      ++st.attempt;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64, SMX_INTEGRATOR_WAVES) void integrate_library_kernel(
     smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
     const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
     const int32_t* __restrict__ bm, const double* __restrict__ v0, const double* __restrict__ sd,
     double* __restrict__ ws, double* __restrict__ valid_out, double* __restrict__ T_out,
     double* __restrict__ tfin_out, double* __restrict__ ckpt, int ckpt_every) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n_int) return;
-  const int P = g.nR * g.nz * g.nphi;
-  const int64_t stride = slab_stride(P, g.num_vbin);
-  double* rz = ws + i * stride;
-  double* ap = rz + slab_rz_len(P);
-  double y[6], k1[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) y[j] = wb[6 * i + j];
-  deriv(p, s.Omega, y, k1);
-  const double EJ0 = jacobi(p, s.Omega, y);
-  double t = 0.0, dt = dt0[i];
-  double* ck = ckpt != nullptr ? ckpt + i * 8 : nullptr;
-  const double T_acc =
-      g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, ck, n_int * 8,
-                                        ckpt_every)
-                 : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, ck, n_int * 8,
-                                        ckpt_every);
-  const double EJ = jacobi(p, s.Omega, y);
-  const bool ok = (fabs(EJ / EJ0 - 1.0) < 0.1) && (t > T_total[i] / 10.0);
-  valid_out[i] = ok ? 1.0 : 0.0;
-  T_out[i] = T_acc + kEps;
-  tfin_out[i] = t;
+  __shared__ double ap_lds[24 * 64];         // aperture-record cache, 4 copies x 6 terms
+  __shared__ double handoff[kHandoff * 64];  // head -> tail states, one per lane
+  const int64_t W = gridDim.x;
+  if (g.symmetry)
+    integrate_balanced<4>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, valid_out, T_out,
+                          tfin_out, ckpt, ckpt_every, ap_lds + threadIdx.x, handoff);
+  else
+    integrate_balanced<1>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, valid_out, T_out,
+                          tfin_out, ckpt, ckpt_every, ap_lds + threadIdx.x, handoff);
+}
+
+// Waves of the balanced integrator: the waves resident at once when there
+// are more integrations than their lanes, else one lane per integration.
+static int64_t integrator_waves(int64_t n_int) {
+  static int resident = 0;
+  static std::mutex mu;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (resident == 0) {
+      int dev = 0, cus = 0, per_cu = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, integrate_library_kernel, 64, 0) ==
+              hipSuccess &&
+          cus > 0 && per_cu > 0)
+        resident = cus * per_cu;
+      else
+        resident = 1024;
+    }
+  }
+  const int64_t lanes = (n_int + 63) / 64;
+  return lanes < resident ? lanes : resident;
 }
 
 // lib[c, :] = sum_r valid_r * (slab_r / T_r) / (sum_r valid_r + 0.1)
 // (integrates.py:212-236, 287-300); output row order
 // [Rzphi | surface density | h1 | h2 | h3 | h4 | h0].
 // One work item per (column, cell) and per (column, aperture): an aperture
-// item reads its 64-byte record once per realization and writes the six
+// item reads its 48-byte record once per realization and writes the six
 // aperture rows (counts, h1..h4, h0), rather than six items each reading one
 // value of the record (whose other values then come back from beyond L2 for
 // the other five).
@@ -165,7 +361,7 @@ __global__ void finalize_library_kernel(const double* __restrict__ ws, int64_t n
   double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < r; ++k) {
     const int64_t it = c * r + k;
-    const double* rec = ws + it * stride + slab_rz_len(P) + 8 * a;
+    const double* rec = ws + it * stride + slab_rz_len(P) + kApRec * a;
     const double T = pre_normalised ? 1.0 : T_int[it];
     const bool ok = valid[it] > 0.5;
 #pragma unroll
@@ -244,7 +440,7 @@ __global__ __launch_bounds__(64) void bin_traj_kernel(
       if (a >= 0) {
         double gw, H[5];
         gh_terms(vlos, v0[a], sd[a], gw, H);
-        double* rec = ap + 8 * (int64_t)a;
+        double* rec = ap + kApRec * (int64_t)a;
         rec[0] += w;
         const double wg = w * gw;
 #pragma unroll
@@ -402,7 +598,8 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
   hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, ws, words, 0.0);
   SMX_LAUNCHED();
   stage_begin(ST_INTEGRATE, st);
-  hipLaunchKernelGGL(integrate_library_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p,
+  hipLaunchKernelGGL(integrate_library_kernel, dim3((unsigned)integrator_waves(n_int)), dim3(64), 0,
+                     st, *p,
                      *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp, ws, valid, T_int,
                      t_final, ckpt, ckpt_every);
   SMX_LAUNCHED();

@@ -151,6 +151,15 @@ int smx_bin_samples(const smx_grids* g, const double* samples, const double* dt,
                     const int32_t* bin_mapping, const double* v0, const double* sdisp,
                     int32_t* idx_rz, int32_t* idx_ap, double* acc_rz, double* acc_ap,
                     void* stream);
+/* Host-only check of the library integrator's work schedule (no counterpart
+ * in the reference, which integrates one orbit per vmap lane): replays, with
+ * the kernel's own schedule arithmetic, how `waves` waves of 64 lanes share
+ * n_int integrations of L attempts (each wave's attempts spread over its
+ * lanes, an integration cut at a lane boundary finished by the next lane).
+ * Returns 0 when every integration makes attempts 0..L-1 exactly once, in
+ * order, and is finished once; *steps_out = the per-lane loop length of
+ * wave 0.  Touches no GPU. */
+int smx_integrator_schedule_check(int64_t n_int, int64_t waves, int64_t L, int64_t* steps_out);
 
 /* ---- density target (model.py:192-213) ---------------------------------- */
 /* Cell masses by Sobol quadrature: out (P) from cell centres (P) x3 and

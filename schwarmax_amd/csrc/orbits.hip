@@ -6,6 +6,7 @@
 #include <cstring>
 #include <cstdarg>
 #include <mutex>
+#include <vector>
 #include <string>
 
 #include "common.hpp"
@@ -122,15 +123,30 @@ struct IntegState {
 };
 constexpr int kHandoff = 16;   // doubles per handed-over state
 
-// One lane's schedule: segments in loop-step order.
+// One lane's schedule: segments in loop-step order.  m integrations of L
+// attempts on the wave's 64 lanes; lane `lane`'s share of the attempts.
 struct LaneSchedule {
   int64_t C, L;
   int64_t jb, hb;          // head: integration jb, attempts [0, hb)
   int64_t jf, nfull;       // whole integrations jf .. jf + nfull - 1 from step hb
   int64_t ja, ta;          // tail: integration ja, attempts [ta, L), steps [C - L + ta, C)
+  int nseg;                // segments of this lane
+  __host__ __device__ void init(int lane, int64_t m, int64_t L_) {
+    L = L_;
+    C = (m * L + 63) / 64 > L ? (m * L + 63) / 64 : L;
+    const int64_t a0 = lane * C < m * L ? lane * C : m * L;
+    const int64_t a1 = (lane + 1) * C < m * L ? (lane + 1) * C : m * L;
+    jb = a1 / L;
+    hb = jb < m ? a1 - jb * L : 0;
+    jf = (a0 + L - 1) / L;
+    nfull = a1 / L - jf > 0 ? a1 / L - jf : 0;
+    ja = a0 / L;
+    ta = ja < m ? a0 - ja * L : 0;
+    nseg = a1 > a0 ? (hb > 0 ? 1 : 0) + (int)nfull + (ta > 0 ? 1 : 0) : 0;
+  }
   // segment s (0 .. nseg-1) -> (kind, integration, first attempt, start step, end step)
-  __device__ void segment(int sidx, int& kind, int64_t& job, int64_t& first, int64_t& t0,
-                          int64_t& t1) const {
+  __host__ __device__ void segment(int sidx, int& kind, int64_t& job, int64_t& first, int64_t& t0,
+                                   int64_t& t1) const {
     int q = sidx;
     if (hb > 0) {
       if (q == 0) { kind = 0; job = jb; first = 0; t0 = 0; t1 = hb; return; }
@@ -140,7 +156,6 @@ struct LaneSchedule {
     q -= (int)nfull;
     kind = 2; job = ja; first = ta; t0 = C - L + ta; t1 = C;   // q == 0 and ta > 0
   }
-  __device__ int count() const { return (hb > 0 ? 1 : 0) + (int)nfull + (ta > 0 ? 1 : 0); }
 };
 
 template <int NC>
@@ -166,17 +181,8 @@ __device__ __forceinline__ void integrate_balanced(
     return;
   }
   LaneSchedule ls;
-  ls.L = L;
-  ls.C = (m * L + 63) / 64 > L ? (m * L + 63) / 64 : L;
-  const int64_t a0 = lane * ls.C < m * L ? lane * ls.C : m * L;
-  const int64_t a1 = (lane + 1) * ls.C < m * L ? (lane + 1) * ls.C : m * L;
-  ls.jb = a1 / L;
-  ls.hb = ls.jb < m ? a1 - ls.jb * L : 0;
-  ls.jf = (a0 + L - 1) / L;
-  ls.nfull = a1 / L - ls.jf > 0 ? a1 / L - ls.jf : 0;
-  ls.ja = a0 / L;
-  ls.ta = ls.ja < m ? a0 - ls.ja * L : 0;
-  const int nseg = a1 > a0 ? ls.count() : 0;
+  ls.init(lane, m, L);
+  const int nseg = ls.nseg;
 
   const int P = g.nR * g.nz * g.nphi;
   const int64_t stride = slab_stride(P, g.num_vbin);
@@ -514,6 +520,73 @@ __global__ __launch_bounds__(256) void density_target_kernel(
 }  // namespace smx
 
 using namespace smx;
+
+// Host replay of the balanced schedule's events (the kernel's loop without
+// the integration): every integration of every wave must make attempts
+// 0 .. L-1 exactly once and in order, a tail must start at or after its head
+// has handed over, no lane may run two segments at once, and every
+// integration is finished once.  Returns 0 when all hold; *steps_out = the
+// loop length C of the first wave.
+int smx_integrator_schedule_check(int64_t n_int, int64_t waves, int64_t L, int64_t* steps_out) {
+  SMX_REQUIRE(n_int >= 0 && waves >= 1 && L >= 1 && steps_out,
+              "smx_integrator_schedule_check: bad arguments");
+  std::vector<int64_t> next(n_int, 0), done(n_int, 0);
+  *steps_out = 0;
+  for (int64_t w = 0; w < waves; ++w) {
+    const int64_t j0 = w * n_int / waves, m = (w + 1) * n_int / waves - j0;
+    if (m <= 0) continue;
+    LaneSchedule ls[64];
+    int sidx[64], kind[64];
+    int64_t job[64], first[64], t0[64], t1[64], cur[64], att[64], hand_job[64], hand_att[64];
+    for (int l = 0; l < 64; ++l) {
+      ls[l].init(l, m, L);
+      sidx[l] = 0;
+      cur[l] = -1;
+      hand_job[l] = -1;
+      if (ls[l].nseg > 0) ls[l].segment(0, kind[l], job[l], first[l], t0[l], t1[l]);
+    }
+    const int64_t C = ls[0].C;
+    if (w == 0) *steps_out = C;
+    for (int64_t step = 0; step <= C; ++step) {
+      bool start[64];
+      for (int l = 0; l < 64; ++l) {
+        const bool ends = sidx[l] < ls[l].nseg && cur[l] >= 0 && step == t1[l];
+        start[l] = sidx[l] < ls[l].nseg && cur[l] < 0 && step == t0[l];
+        if (ends) {
+          if (kind[l] == 0) {
+            hand_job[l] = cur[l];
+            hand_att[l] = att[l];
+          } else {
+            if (att[l] != L || done[j0 + cur[l]]++ != 0) return 1;
+          }
+          cur[l] = -1;
+          if (++sidx[l] < ls[l].nseg) {
+            ls[l].segment(sidx[l], kind[l], job[l], first[l], t0[l], t1[l]);
+            start[l] = step == t0[l];
+          }
+        }
+      }
+      for (int l = 0; l < 64; ++l) {
+        if (!start[l]) continue;
+        if (kind[l] == 2 && (l == 0 || hand_job[l - 1] != job[l] || hand_att[l - 1] != first[l]))
+          return 2;
+        cur[l] = job[l];
+        att[l] = first[l];
+      }
+      for (int l = 0; l < 64; ++l) {
+        if (cur[l] < 0 || step >= C) continue;
+        int64_t& nx = next[j0 + cur[l]];
+        if (att[l] != nx) return 3;
+        ++nx;
+        ++att[l];
+      }
+    }
+  }
+  for (int64_t i = 0; i < n_int; ++i)
+    if (next[i] != L || done[i] != 1) return 4;
+  return 0;
+}
+
 
 extern "C" {
 

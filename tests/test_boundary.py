@@ -37,7 +37,7 @@ def test_workspace_queries_without_gpu():
     g = M.make_grids(100, (8, 4, 6), [[0, 10], [-3, 3], [-np.pi, np.pi]], [[-5, 5], [-5, 5]],
                      (20, 20), np.eye(3), True)
     P = 8 * 4 * 6
-    assert L.smx_library_workspace_bytes(g, 10) == 10 * 8 * (((P + 7) // 8) * 8 + 8 * 100)
+    assert L.smx_library_workspace_bytes(g, 10) == 10 * 8 * (((P + 7) // 8) * 8 + 6 * 100)
     assert L.smx_nnls_workspace_bytes(1000, 700, 20) > 4 * 1000 * 1000
     assert L.smx_nnls_workspace_bytes(0, 1, 1) == -1
 

@@ -1,6 +1,7 @@
 """The ADMM product kernel (admm_gemm_kernel, through smx_nnls_operator_apply
 and smx_nnls_operator_apply_hi) against R M in FP64, at shapes that exercise
-its tiling: one and two 128-row blocks of R (K = 1, 100, 130), splits of odd
+its tiling: one, two and three 128-row blocks of R (K = 1, 100, 130, 200,
+256, 300; more than one block reads M with plain loads), splits of odd
 and even chunk counts, a split of a single chunk and empty splits (n = 1,100:
 40 chunks of 32 over 16 splits), zero padding of n to a multiple of 256, and
 three occupancy settings (which change the split count).
@@ -24,6 +25,7 @@ def _rowwise(got, ref):
 
 
 @pytest.mark.parametrize("n,m,K,occ", [(1100, 700, 1, 2), (1100, 700, 100, 2), (1100, 700, 130, 2),
+                                       (1100, 700, 200, 4), (1100, 700, 300, 2), (3000, 1500, 256, 1),
                                        (5000, 3000, 100, 1), (5000, 3000, 100, 2),
                                        (4100, 2500, 60, 4)])
 def test_product_matches_fp64(n, m, K, occ):

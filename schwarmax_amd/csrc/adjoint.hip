@@ -19,6 +19,7 @@
 //    potential fields (dual numbers seeded a few fields per pass).
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 #include "potential.hpp"
@@ -220,68 +221,94 @@ __device__ __forceinline__ double& segv(double* seg, int64_t n_int, int64_t i, i
   return seg[((int64_t)a * SEG_W + j) * n_int + i];
 }
 
-__global__ __launch_bounds__(64) void library_vjp_kernel(
-    smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
-    const double* __restrict__ dt0, int64_t n_int, int r, const int32_t* __restrict__ bm,
-    const double* __restrict__ npb, const double* __restrict__ v0, const double* __restrict__ sd,
-    const double* __restrict__ gbar, const double* __restrict__ scale,
-    const double* __restrict__ tbar, const double* __restrict__ ckpt, int every,
-    double* __restrict__ seg, double* __restrict__ ybar0, double* __restrict__ dtbar0,
-    double* __restrict__ fbar, double* __restrict__ ombar, double* __restrict__ rotbar) {
-  // Every lane of a wave runs the same sequence (lanes past n_int redo the
-  // last integration, invalid integrations run with scale 0 and yield 0), so
-  // deriv_vjp is only ever called with all lanes active.
-  const int64_t i_raw = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool live = i_raw < n_int;
-  const int64_t i = live ? i_raw : n_int - 1;
+// Adjoint state of one integration between units: the cotangents of the
+// state y and FSAL slope k1 before the unit's first attempt, of dt, and the
+// accumulated cotangents of Omega, the potential fields and the rotation.
+struct VjpState {
+  double yb[6], k1b[6], dtb, omb, fb[PF_COUNT], rotb[3];
+};
+constexpr int kVjpHand = 12 + 2 + PF_COUNT + 3;   // doubles handed from a head to its tail
+
+__device__ __forceinline__ void vjp_zero(VjpState& v) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) v.yb[j] = v.k1b[j] = 0.0;
+  v.dtb = v.omb = 0.0;
+#pragma unroll
+  for (int f = 0; f < PF_COUNT; ++f) v.fb[f] = 0.0;
+  v.rotb[0] = v.rotb[1] = v.rotb[2] = 0.0;
+}
+
+// One unit of the adjoint: segment sg of integration i (every attempts from
+// its checkpoint), replayed into i's buffer and swept backwards.  own: this
+// lane owns (i, sg) at this step, replays it and applies the sweep to v.
+// Otherwise the lane shadows (i, sg), which another lane of the wave owns at
+// the same step: it reads the buffer that lane writes, with zero cotangents
+// and scale, so every cotangent it forms is exactly zero and v is untouched.
+// Every lane runs all `every` steps of the sweep (attempts past a short last
+// segment re-read its last attempt with zero cotangents), so the three
+// deriv_vjp calls of each step are made with all lanes of the wave active.
+__device__ __forceinline__ void vjp_unit(const smx_potential& p, const smx_grids& g,
+                                         const smx_schedule& s, int64_t n_int, int r,
+                                         const int32_t* __restrict__ bm,
+                                         const double* __restrict__ npb,
+                                         const double* __restrict__ v0,
+                                         const double* __restrict__ sd,
+                                         const double* __restrict__ gbar,
+                                         const double* __restrict__ scale,
+                                         const double* __restrict__ tbar,
+                                         const double* __restrict__ ckpt, int every,
+                                         double* __restrict__ seg, int64_t i, int sg, bool own,
+                                         VjpState& v) {
   const int P = g.nR * g.nz * g.nphi;
   const int B = g.num_vbin;
   const int nsym = g.symmetry ? 4 : 1;
   const double Om = s.Omega;
-  const double sc = scale[i];
-  double yb[6] = {0, 0, 0, 0, 0, 0}, k1b[6] = {0, 0, 0, 0, 0, 0};
-  double dtb = 0.0, omb = 0.0;
-  double fb[PF_COUNT];
+  const double sc = own ? scale[i] : 0.0;
+  const double Tb = own ? tbar[i] : 0.0;
+  const double* gr = gbar + (i / r) * ((int64_t)P + 6 * (int64_t)B);
+  const int a0 = sg * every;
+  const int na = (s.N_max - a0 < every) ? s.N_max - a0 : every;
+  // replay the segment from its checkpoint (owner only: no out-of-line calls)
+  if (own) {
+    const double* c = ckpt + ((int64_t)sg * n_int + i) * 8;
+    double y[6], k1[6], dt = c[6], t = c[7];
 #pragma unroll
-  for (int f = 0; f < PF_COUNT; ++f) fb[f] = 0.0;
-  double rotb[3] = {0.0, 0.0, 0.0};
-  {
-    const double Tb = tbar[i];
-    const double* gr = gbar + (i / r) * ((int64_t)P + 6 * (int64_t)B);
-    const int nseg = (s.N_max + every - 1) / every;
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-      const int a0 = sg * every;
-      const int na = (s.N_max - a0 < every) ? s.N_max - a0 : every;
-      // replay the segment from its checkpoint
-      {
-        const double* c = ckpt + ((int64_t)sg * n_int + i) * 8;
-        double y[6], k1[6], dt = c[6], t = c[7];
+    for (int j = 0; j < 6; ++j) y[j] = c[j];
+    deriv(p, Om, y, k1);
+    for (int a = 0; a < na; ++a) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) y[j] = c[j];
-        deriv(p, Om, y, k1);
-        for (int a = 0; a < na; ++a) {
-#pragma unroll
-          for (int j = 0; j < 6; ++j) {
-            if (live) {
-              segv(seg, n_int, i, a, j) = y[j];
-              segv(seg, n_int, i, a, 7 + j) = k1[j];
-            }
-          }
-          if (live) segv(seg, n_int, i, a, 6) = dt;
-          double k2[6], k3[6], k4[6];
-          bs3_attempt_stages(p, s, t, y, k1, dt, k2, k3, k4);
-#pragma unroll
-          for (int j = 0; j < 6; ++j) {
-            if (live) {
-              segv(seg, n_int, i, a, 13 + j) = k2[j];
-              segv(seg, n_int, i, a, 19 + j) = k3[j];
-              segv(seg, n_int, i, a, 25 + j) = k4[j];
-            }
-          }
-        }
+      for (int j = 0; j < 6; ++j) {
+        segv(seg, n_int, i, a, j) = y[j];
+        segv(seg, n_int, i, a, 7 + j) = k1[j];
       }
-      // reverse sweep
-      for (int a = na - 1; a >= 0; --a) {
+      segv(seg, n_int, i, a, 6) = dt;
+      double k2[6], k3[6], k4[6];
+      bs3_attempt_stages(p, s, t, y, k1, dt, k2, k3, k4);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        segv(seg, n_int, i, a, 13 + j) = k2[j];
+        segv(seg, n_int, i, a, 19 + j) = k3[j];
+        segv(seg, n_int, i, a, 25 + j) = k4[j];
+      }
+    }
+  }
+  // the owner's buffer writes above are read below by lanes shadowing i
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  // reverse sweep
+  for (int a_step = every - 1; a_step >= 0; --a_step) {
+    const bool act = own && a_step < na;
+    const int a = a_step < na ? a_step : na - 1;
+    double yb[6], k1b[6], dtb;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      yb[j] = act ? v.yb[j] : 0.0;
+      k1b[j] = act ? v.k1b[j] : 0.0;
+    }
+    dtb = act ? v.dtb : 0.0;
+    double& omb = v.omb;
+    double* fb = v.fb;
+    double* rotb = v.rotb;
+    {
         double y[6], k1[6], k2[6], k3[6], k4[6];
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
@@ -319,7 +346,7 @@ __global__ __launch_bounds__(64) void library_vjp_kernel(
 #pragma unroll
           for (int j = 0; j < 6; ++j) ynb[j] = yb[j];
           // the sample y_{a+1} = yn binned with weight dt; T += nsym dt
-          const double dw = bin_vjp(g, bm, v0, sd, npb, gr, yn, dt, sc, nsym, P, ynb, rotb);
+          const double dw = act ? bin_vjp(g, bm, v0, sd, npb, gr, yn, dt, sc, nsym, P, ynb, rotb) : 0.0;
           dtA += sc * dw + nsym * Tb;
         } else {
 #pragma unroll
@@ -402,22 +429,176 @@ __global__ __launch_bounds__(64) void library_vjp_kernel(
           k1b[j] = k1l[j];
         }
         dtb = dtA;
+    }
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        v.yb[j] = yb[j];
+        v.k1b[j] = k1b[j];
+      }
+      v.dtb = dtb;
+    }
+  }
+}
+
+// Balanced over the resident waves like the library integrator: each wave
+// takes the contiguous share of m integrations [w n / W, (w + 1) n / W) and
+// spreads their m U units (U = ceil(N_max / every) segments each, swept from
+// the last segment to the first) over its lanes by LaneSchedule; a split
+// integration's adjoint state passes from the lane that sweeps its last
+// segments to the lane that sweeps the rest through LDS.  A lane without a
+// unit at a step shadows lane 0's unit (lane 0 is busy at every step).  After
+// each step every lane makes one more deriv_vjp call: k1_0 = f(y_0) of an
+// integration finished at that step, with zero cotangent elsewhere.
+__global__ __launch_bounds__(64) void library_vjp_kernel(
+    smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
+    const double* __restrict__ dt0, int64_t n_int, int r, const int32_t* __restrict__ bm,
+    const double* __restrict__ npb, const double* __restrict__ v0, const double* __restrict__ sd,
+    const double* __restrict__ gbar, const double* __restrict__ scale,
+    const double* __restrict__ tbar, const double* __restrict__ ckpt, int every,
+    double* __restrict__ seg, double* __restrict__ ybar0, double* __restrict__ dtbar0,
+    double* __restrict__ fbar, double* __restrict__ ombar, double* __restrict__ rotbar) {
+  __shared__ double handoff[kVjpHand * 64];
+  const int lane = threadIdx.x;
+  const int64_t W = gridDim.x, w = blockIdx.x;
+  const int64_t j0 = w * n_int / W, m = (w + 1) * n_int / W - j0;
+  const int64_t U = (s.N_max + every - 1) / every;
+  if (m <= 0) return;
+  const double Om = s.Omega;
+  if (U <= 0) {
+    // no attempts: the cotangents are those of k1_0 = f(y_0) with k1b = 0
+    for (int64_t q = lane; q < m; q += 64) {
+      const int64_t i = j0 + q;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) ybar0[6 * i + j] = 0.0;
+      dtbar0[i] = 0.0;
+      for (int f = 0; f < PF_COUNT; ++f) fbar[i * PF_COUNT + f] = 0.0;
+      ombar[i] = 0.0;
+      for (int j = 0; j < 3; ++j) rotbar[3 * i + j] = 0.0;
+    }
+    return;
+  }
+  LaneSchedule ls, l0;
+  ls.init(lane, m, U);
+  l0.init(0, m, U);
+  const int nseg = ls.nseg;
+  int sidx = 0, kind = 0, k0 = 0;
+  int64_t job = 0, first = 0, t0 = 0, t1 = 0;
+  if (nseg > 0) ls.segment(0, kind, job, first, t0, t1);
+  int s0 = 0, kind0 = 0;
+  int64_t job0 = 0, first0 = 0, a00 = 0, a01 = 0;
+  l0.segment(0, kind0, job0, first0, a00, a01);
+  (void)k0;
+  VjpState v;
+  vjp_zero(v);
+  int64_t cur = -1;    // integration this lane holds, or -1
+  int64_t unit = 0;    // its next unit (0 = its last segment)
+  for (int64_t step = 0; step < ls.C; ++step) {
+    // lane 0's unit at this step (every lane follows lane 0's schedule)
+    while (step >= a01) {
+      ++s0;
+      l0.segment(s0, kind0, job0, first0, a00, a01);
+    }
+    const int64_t u0 = first0 + (step - a00);
+    // start of this lane's next segment
+    if (sidx < nseg && cur < 0 && step == t0) {
+      cur = job;
+      unit = first;
+      if (kind == 2) {
+        const double* h = handoff + (lane - 1);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          v.yb[j] = h[j * 64];
+          v.k1b[j] = h[(6 + j) * 64];
+        }
+        v.dtb = h[12 * 64];
+        v.omb = h[13 * 64];
+#pragma unroll
+        for (int f = 0; f < PF_COUNT; ++f) v.fb[f] = h[(14 + f) * 64];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v.rotb[j] = h[(14 + PF_COUNT + j) * 64];
+      } else {
+        vjp_zero(v);
       }
     }
-    // k1_0 = f(y_0)
-    double y0[6];
+    const bool own = cur >= 0;
+    const int64_t i = j0 + (own ? cur : job0);
+    const int sgi = (int)(U - 1 - (own ? unit : u0));
+    vjp_unit(p, g, s, n_int, r, bm, npb, v0, sd, gbar, scale, tbar, ckpt, every, seg, i, sgi, own,
+             v);
+    bool finished = false;
+    if (own) {
+      ++unit;
+      if (step + 1 == t1) {   // end of this lane's segment
+        if (kind == 0) {
+          double* h = handoff + lane;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) y0[j] = wb[6 * i + j];
-    deriv_vjp(p, Om, y0, k1b, yb, omb, fb);
+          for (int j = 0; j < 6; ++j) {
+            h[j * 64] = v.yb[j];
+            h[(6 + j) * 64] = v.k1b[j];
+          }
+          h[12 * 64] = v.dtb;
+          h[13 * 64] = v.omb;
+#pragma unroll
+          for (int f = 0; f < PF_COUNT; ++f) h[(14 + f) * 64] = v.fb[f];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) h[(14 + PF_COUNT + j) * 64] = v.rotb[j];
+          cur = -1;
+        } else {
+          finished = true;
+        }
+        ++sidx;
+        if (sidx < nseg) ls.segment(sidx, kind, job, first, t0, t1);
+      }
+    }
+    // k1_0 = f(y_0) for an integration finished at this step; zero cotangent
+    // (at a valid state) elsewhere
+    {
+      const int64_t ie = finished ? i : j0 + job0;
+      double y0[6], k1z[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        y0[j] = wb[6 * ie + j];
+        k1z[j] = finished ? v.k1b[j] : 0.0;
+      }
+      deriv_vjp(p, Om, y0, k1z, v.yb, v.omb, v.fb);
+    }
+    if (finished) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) ybar0[6 * i + j] = v.yb[j];
+      dtbar0[i] = v.dtb;
+#pragma unroll
+      for (int f = 0; f < PF_COUNT; ++f) fbar[i * PF_COUNT + f] = v.fb[f];
+      ombar[i] = v.omb;
+      for (int j = 0; j < 3; ++j) rotbar[3 * i + j] = v.rotb[j];
+      cur = -1;
+      vjp_zero(v);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");   // hand-overs before later reads
   }
-  if (!live) return;
-#pragma unroll
-  for (int j = 0; j < 6; ++j) ybar0[6 * i + j] = yb[j];
-  dtbar0[i] = dtb;
-#pragma unroll
-  for (int f = 0; f < PF_COUNT; ++f) fbar[i * PF_COUNT + f] = fb[f];
-  ombar[i] = omb;
-  for (int j = 0; j < 3; ++j) rotbar[3 * i + j] = rotb[j];
+}
+
+// Waves of the balanced adjoint: the waves resident at once when there are
+// more integrations than their lanes, else one lane per integration.
+static int64_t vjp_waves(int64_t n_int) {
+  static int resident = 0;
+  static std::mutex mu;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (resident == 0) {
+      int dev = 0, cus = 0, per_cu = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, library_vjp_kernel, 64, 0) ==
+              hipSuccess &&
+          cus > 0 && per_cu > 0)
+        resident = cus * per_cu;
+      else
+        resident = 1024;
+    }
+  }
+  const int64_t lanes = (n_int + 63) / 64;
+  return lanes < resident ? lanes : resident;
 }
 
 // ---------------------------------------------------------------------------
@@ -561,7 +742,7 @@ int smx_library_vjp(const smx_potential* p, const smx_grids* g, const smx_schedu
                      static_cast<const double*>(fwd_workspace), n_int, (int)r, P, g->num_vbin, valid,
                      T_int, num_per_bin, g->area_pixel, lib_bar, scale, tbar);
   SMX_LAUNCHED();
-  hipLaunchKernelGGL(library_vjp_kernel, dim3(blocks_for(n_int, 64)), dim3(64), 0, st, *p, *g, *s,
+  hipLaunchKernelGGL(library_vjp_kernel, dim3((unsigned)vjp_waves(n_int)), dim3(64), 0, st, *p, *g, *s,
                      wb, dt0, n_int, (int)r, bin_mapping, num_per_bin, v0, sdisp, lib_bar, scale,
                      tbar, ckpt, (int)ckpt_every, seg, wb_bar, dt0_bar, field_bar, omega_bar,
                      rot_bar);

@@ -272,4 +272,47 @@ __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, cons
   return T_acc;
 }
 
+// Work schedule of the balanced integrator and adjoint kernels (McNaughton's
+// wrap-around rule within one wave): m jobs of L equal units each, spread over
+// the wave's 64 lanes.  Lane l covers units [l C, (l + 1) C) of the jobs'
+// concatenation, C = max(L, ceil(m L / 64)); a job cut at a lane boundary
+// runs its first units (its head) on lane l at the start of the loop and the
+// rest (its tail) on lane l + 1 at the end, ending at step C, so the parts
+// never overlap (C >= L).
+// One lane's schedule: segments in loop-step order.  m integrations of L
+// attempts on the wave's 64 lanes; lane `lane`'s share of the attempts.
+struct LaneSchedule {
+  int64_t C, L;
+  int64_t jb, hb;          // head: integration jb, attempts [0, hb)
+  int64_t jf, nfull;       // whole integrations jf .. jf + nfull - 1 from step hb
+  int64_t ja, ta;          // tail: integration ja, attempts [ta, L), steps [C - L + ta, C)
+  int nseg;                // segments of this lane
+  __host__ __device__ void init(int lane, int64_t m, int64_t L_) {
+    L = L_;
+    C = (m * L + 63) / 64 > L ? (m * L + 63) / 64 : L;
+    const int64_t a0 = lane * C < m * L ? lane * C : m * L;
+    const int64_t a1 = (lane + 1) * C < m * L ? (lane + 1) * C : m * L;
+    jb = a1 / L;
+    hb = jb < m ? a1 - jb * L : 0;
+    jf = (a0 + L - 1) / L;
+    nfull = a1 / L - jf > 0 ? a1 / L - jf : 0;
+    ja = a0 / L;
+    ta = ja < m ? a0 - ja * L : 0;
+    nseg = a1 > a0 ? (hb > 0 ? 1 : 0) + (int)nfull + (ta > 0 ? 1 : 0) : 0;
+  }
+  // segment s (0 .. nseg-1) -> (kind, integration, first attempt, start step, end step)
+  __host__ __device__ void segment(int sidx, int& kind, int64_t& job, int64_t& first, int64_t& t0,
+                                   int64_t& t1) const {
+    int q = sidx;
+    if (hb > 0) {
+      if (q == 0) { kind = 0; job = jb; first = 0; t0 = 0; t1 = hb; return; }
+      --q;
+    }
+    if (q < nfull) { kind = 1; job = jf + q; first = 0; t0 = hb + q * L; t1 = t0 + L; return; }
+    q -= (int)nfull;
+    kind = 2; job = ja; first = ta; t0 = C - L + ta; t1 = C;   // q == 0 and ta > 0
+  }
+};
+
+
 }  // namespace smx

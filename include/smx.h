@@ -160,6 +160,14 @@ int smx_bin_samples(const smx_grids* g, const double* samples, const double* dt,
  * order, and is finished once; *steps_out = the per-lane loop length of
  * wave 0.  Touches no GPU. */
 int smx_integrator_schedule_check(int64_t n_int, int64_t waves, int64_t L, int64_t* steps_out);
+/* The same for the balanced library adjoint (no counterpart in the
+ * reference, which differentiates with jax.grad): replays its step loop over
+ * U checkpoint segments per integration, swept from the last, with the
+ * lanes that have no unit shadowing lane 0.  Returns 0 when every
+ * integration's units are swept once and in order, every tail starts from
+ * its head's hand-over, lane 0 holds the shadowed unit at every step and
+ * every integration is finished once.  Touches no GPU. */
+int smx_adjoint_schedule_check(int64_t n_int, int64_t waves, int64_t U, int64_t* steps_out);
 
 /* ---- density target (model.py:192-213) ---------------------------------- */
 /* Cell masses by Sobol quadrature: out (P) from cell centres (P) x3 and

@@ -110,6 +110,7 @@ _SIGS = [
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ('smx_integrator_schedule_check', ctypes.c_int, [_I64, _I64, _I64, _P]),
+    ('smx_adjoint_schedule_check', ctypes.c_int, [_I64, _I64, _I64, _P]),
     ('smx_bin_samples', ctypes.c_int,
      [ctypes.POINTER(SmxGrids), _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
     ('smx_density_target', ctypes.c_int,

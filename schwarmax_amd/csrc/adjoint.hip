@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "potential.hpp"
@@ -711,6 +712,79 @@ constexpr int JAC_SLOTS = 4;
 using namespace smx;
 
 extern "C" {
+
+// Host replay of library_vjp_kernel's step loop (segment starts and ends,
+// hand-overs, shadows), with the kernel's own schedule: every integration
+// must have its units 0 .. U-1 swept once and in order, a tail must start
+// from the state its head handed over, a lane without a unit must find lane
+// 0 on the unit it shadows, and every integration is finished once.
+// Returns 0 when all hold; *steps_out = the loop length of wave 0.
+int smx_adjoint_schedule_check(int64_t n_int, int64_t waves, int64_t U, int64_t* steps_out) {
+  SMX_REQUIRE(n_int >= 0 && waves >= 1 && U >= 1 && steps_out,
+              "smx_adjoint_schedule_check: bad arguments");
+  std::vector<int64_t> next(n_int, 0), done(n_int, 0);
+  *steps_out = 0;
+  for (int64_t w = 0; w < waves; ++w) {
+    const int64_t j0 = w * n_int / waves, m = (w + 1) * n_int / waves - j0;
+    if (m <= 0) continue;
+    LaneSchedule ls[64], l0;
+    int sidx[64], kind[64];
+    int64_t job[64], first[64], t0[64], t1[64], cur[64], unit[64], hand_job[64], hand_unit[64];
+    for (int l = 0; l < 64; ++l) {
+      ls[l].init(l, m, U);
+      sidx[l] = 0;
+      cur[l] = -1;
+      hand_job[l] = -1;
+      if (ls[l].nseg > 0) ls[l].segment(0, kind[l], job[l], first[l], t0[l], t1[l]);
+    }
+    l0.init(0, m, U);
+    int s0 = 0, kind0 = 0;
+    int64_t job0 = 0, first0 = 0, a00 = 0, a01 = 0;
+    l0.segment(0, kind0, job0, first0, a00, a01);
+    const int64_t C = ls[0].C;
+    if (w == 0) *steps_out = C;
+    for (int64_t step = 0; step < C; ++step) {
+      while (step >= a01) {
+        if (++s0 >= l0.nseg) return 5;
+        l0.segment(s0, kind0, job0, first0, a00, a01);
+      }
+      const int64_t u0 = first0 + (step - a00);
+      int64_t lane0_job = -2, lane0_unit = -2;
+      for (int l = 0; l < 64; ++l) {
+        if (sidx[l] < ls[l].nseg && cur[l] < 0 && step == t0[l]) {
+          cur[l] = job[l];
+          unit[l] = first[l];
+          if (kind[l] == 2 &&
+              (l == 0 || hand_job[l - 1] != job[l] || hand_unit[l - 1] != first[l]))
+            return 2;
+        }
+        if (cur[l] < 0) continue;
+        int64_t& nx = next[j0 + cur[l]];
+        if (unit[l] != nx) return 3;
+        if (l == 0) {
+          lane0_job = cur[l];
+          lane0_unit = unit[l];
+        }
+        ++nx;
+        ++unit[l];
+        if (step + 1 == t1[l]) {
+          if (kind[l] == 0) {
+            hand_job[l] = cur[l];
+            hand_unit[l] = unit[l];
+          } else if (unit[l] != U || done[j0 + cur[l]]++ != 0) {
+            return 1;
+          }
+          cur[l] = -1;
+          if (++sidx[l] < ls[l].nseg) ls[l].segment(sidx[l], kind[l], job[l], first[l], t0[l], t1[l]);
+        }
+      }
+      if (lane0_job != job0 || lane0_unit != u0) return 6;   // the shadows' unit
+    }
+  }
+  for (int64_t i = 0; i < n_int; ++i)
+    if (next[i] != U || done[i] != 1) return 4;
+  return 0;
+}
 
 int64_t smx_library_vjp_workspace_bytes(int64_t n_int, int32_t ckpt_every) {
   if (n_int < 0 || ckpt_every < 1) return -1;

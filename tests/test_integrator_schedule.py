@@ -1,5 +1,6 @@
-"""The library integrator's work schedule, replayed on the host by the
-library's own schedule arithmetic (`smx_integrator_schedule_check`; no GPU).
+"""The library integrator's and adjoint's work schedules, replayed on the host
+by the library's own schedule arithmetic (`smx_integrator_schedule_check`,
+`smx_adjoint_schedule_check`; no GPU).
 
 The kernel runs as many waves as are resident at once and spreads each wave's
 share of integrations over its 64 lanes (McNaughton's wrap-around rule); an
@@ -49,3 +50,25 @@ def test_plain20k_loop_length():
 def test_bad_arguments_are_rejected():
     rc, _ = _check(10, 0, 5)
     assert rc != 0
+
+
+def _check_adjoint(n_int, waves, U):
+    lib = _lib.load_library()
+    steps = ctypes.c_int64(0)
+    rc = lib.smx_adjoint_schedule_check(n_int, waves, U, ctypes.byref(steps))
+    return rc, steps.value
+
+
+@pytest.mark.parametrize("n_int,waves,U", [
+    (80_000, 1_024, 32),       # grad-20k: 2,000 attempts in segments of 64
+    (80, 1, 10), (65, 1, 9), (64, 1, 9), (127, 2, 3), (300, 4, 1), (1_000, 3, 13),
+    (7_000, 100, 11), (0, 8, 5), (10, 16, 4),
+])
+def test_adjoint_schedule_replay(n_int, waves, U):
+    """The balanced adjoint's step loop: units swept once and in order,
+    hand-overs read after they are written, lane 0 on the shadowed unit."""
+    rc, steps = _check_adjoint(n_int, waves, U)
+    assert rc == 0, rc
+    m = n_int // waves
+    if m > 0:
+        assert steps == max(U, -(-m * U // 64))

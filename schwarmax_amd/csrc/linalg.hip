@@ -431,6 +431,12 @@ __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t 
 // with that row's power-of-two scale: max |R_k| -> 2^14, so hi + lo = R *
 // scale to 2^-22 relative.  Rows k >= K (zero padding) read row K - 1
 // instead, one cache line shared by the wave, and stage zeros (scale 0).
+// Column (in halfs) of 16-byte piece q (0..3) of row `row` in an R stage of
+// 64-byte rows: the pieces are XOR-swizzled by bits 2..3 of the row, so that
+// the 16 lanes of a fragment read (rows c .. c + 15, one piece each) and the
+// 32 lanes of a staging store (4 rows x 64 bytes) each cover all 64 banks.
+__device__ __forceinline__ int stage_col(int row, int q) { return 8 * (q ^ ((row >> 2) & 3)); }
+
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 struct AdmmA {
   float4 v[4];
@@ -461,8 +467,10 @@ struct AdmmA {
         h[e] = static_cast<_Float16>(f[e]);
         if (FULL) l[e] = static_cast<_Float16>(f[e] - static_cast<float>(h[e]));
       }
-      *reinterpret_cast<half4*>(&Ah[32 * u + t / 8][4 * (t % 8)]) = h;
-      if (FULL) *reinterpret_cast<half4*>(&Al[32 * u + t / 8][4 * (t % 8)]) = l;
+      const int row = 32 * u + t / 8;
+      const int col = stage_col(row, (t % 8) >> 1) + 4 * (t & 1);
+      *reinterpret_cast<half4*>(&Ah[row][col]) = h;
+      if (FULL) *reinterpret_cast<half4*>(&Al[row][col]) = l;
     }
   }
 };
@@ -508,9 +516,10 @@ __device__ __forceinline__ void admm_mfma_chunk(const _Float16 (*Ah)[P], const _
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-      const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][16 * t + 8 * g]);
+      const int col = stage_col(rb * 32 + c, 2 * t + g);
+      const half8 ah = *reinterpret_cast<const half8*>(&Ah[rb * 32 + c][col]);
       half8 al;
-      if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][16 * t + 8 * g]);
+      if (FULL) al = *reinterpret_cast<const half8*>(&Al[rb * 32 + c][col]);
 #pragma unroll
       for (int cb = 0; cb < AG_WCB; ++cb) {
         acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b.h[cb][t], acc[rb][cb], 0, 0, 0);
@@ -598,7 +607,7 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
     const unsigned* __restrict__ dmax, const unsigned* __restrict__ fmax_, int force, float tau,
     const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml, float* __restrict__ part) {
   static_assert(KC == AG_KP, "the blocked layouts use 32-index chunks");
-  constexpr int P = KC + 8;   // LDS row pitch (halfs): conflict-free fragment reads
+  constexpr int P = KC;   // LDS row pitch (halfs): 64 bytes, 16-byte pieces swizzled (stage_col)
   __shared__ __align__(16) _Float16 Ah[2][AG_KR][P];
   __shared__ __align__(16) _Float16 Al[2][AG_KR][P];
   const bool full = admm_full_product(force, K, fmax_, dmax, tau);

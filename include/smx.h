@@ -292,6 +292,39 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m,
 int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* workspace,
                                 int32_t* full_products, void* stream);
 
+/* Grouped bootstrap NNLS: the solves of several models with the same m and K
+ * (the per-theta solve_nnls_admm_bootstrap calls, nnls.py:68-166, that the
+ * reference's jax.vmap over theta puts in one program; example_schwarmax.ipynb
+ * vmaps model_likelihood, likelihood.py:96-244).  Each model's Gram, inverse
+ * and right-hand sides are formed one after another on a shared scratch;
+ * then every ADMM iteration is one product launch and one update launch for
+ * the whole group.  Each model's result is bit-identical to its
+ * smx_nnls_admm call (tol = 0).  A problem holds what smx_nnls_admm takes for
+ * its model; `state` is a device buffer of smx_nnls_group_state_bytes(n, m,
+ * K) bytes that the solve keeps for that model. */
+#define SMX_NNLS_GROUP_MAX 64
+typedef struct smx_nnls_problem {
+  const double* lib;        /* (n, ld_lib) library rows, first m entries the design rows */
+  int64_t ld_lib;
+  int64_t n;
+  const double* row_scale;  /* (m) */
+  const double* Y;          /* (K, m) */
+  double w_init;            /* start w = z (nnls.py:123) */
+  void* state;              /* smx_nnls_group_state_bytes(n, m, K) device bytes */
+  double* W;                /* (K, n) out */
+} smx_nnls_problem;
+int64_t smx_nnls_group_state_bytes(int64_t n, int64_t m, int64_t K);
+/* Scratch shared by a group whose largest model has n_max orbits. */
+int64_t smx_nnls_group_scratch_bytes(int64_t n_max, int64_t m, int64_t K);
+/* n_models in 1 .. SMX_NNLS_GROUP_MAX; problems is a host array.  Runs
+ * maxiter iterations for every model (no early exit). */
+int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* problems, int64_t m, int64_t K,
+                        double lambda_reg, int32_t maxiter, void* scratch, void* stream);
+/* Full products taken by the last grouped solve of the model whose state
+ * buffer this is.  Synchronises the stream. */
+int smx_nnls_group_full_products(int64_t n, int64_t m, int64_t K, const void* state,
+                                 int32_t* full_products, void* stream);
+
 /* ---- ADMM product tuning (no reference counterpart: the knob the
  * autotuner in schwarmax_amd/tune.py sets) ------------------------------- */
 /* 256-thread workgroups per CU that the ADMM product's split-K rule targets

@@ -50,6 +50,17 @@ class SmxSchedule(ctypes.Structure):
     ]
 
 
+class SmxNnlsProblem(ctypes.Structure):
+    _fields_ = [
+        ('lib', ctypes.c_void_p), ('ld_lib', ctypes.c_int64), ('n', ctypes.c_int64),
+        ('row_scale', ctypes.c_void_p), ('Y', ctypes.c_void_p), ('w_init', ctypes.c_double),
+        ('state', ctypes.c_void_p), ('W', ctypes.c_void_p),
+    ]
+
+
+NNLS_GROUP_MAX = 64
+
+
 class SmxCylspline(ctypes.Structure):
     _fields_ = [
         ('u', ctypes.c_void_p), ('v', ctypes.c_void_p), ('coef', ctypes.c_void_p),
@@ -119,6 +130,11 @@ _SIGS = [
     ('smx_nnls_admm', ctypes.c_int,
      [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
     ('smx_nnls_admm_full_products', ctypes.c_int, [_I64, _I64, _I64, _P, ctypes.POINTER(_I32), _P]),
+    ('smx_nnls_group_state_bytes', _I64, [_I64, _I64, _I64]),
+    ('smx_nnls_group_scratch_bytes', _I64, [_I64, _I64, _I64]),
+    ('smx_nnls_admm_group', ctypes.c_int,
+     [_I32, ctypes.POINTER(SmxNnlsProblem), _I64, _I64, ctypes.c_double, _I32, _P, _P]),
+    ('smx_nnls_group_full_products', ctypes.c_int, [_I64, _I64, _I64, _P, ctypes.POINTER(_I32), _P]),
     ('smx_admm_set_occupancy', ctypes.c_int, [_I32]),
     ('smx_admm_get_occupancy', ctypes.c_int, [ctypes.POINTER(_I32)]),
     ('smx_admm_set_delta_threshold', ctypes.c_int, [ctypes.c_double]),

@@ -8,9 +8,13 @@ count n_m (the first n_m orbits of dict_data), runs as:
   libraries of all models in one persistent launch fed by a device work queue
   (``smx_build_library_batch``), on a side stream, group by group under a
   library-memory budget;
-* per model, on the main stream, the density target, bootstrap NNLS (all K
-  right-hand sides batched) and the chi^2, overlapping the next group's
-  integration: the integrator is FP64-VALU bound, the NNLS MFMA/HBM bound.
+* on the main stream, overlapping the next group's integration (the
+  integrator is FP64-VALU bound, the NNLS MFMA/HBM bound): per model the
+  density target and the bootstrapped data, then the bootstrap NNLS of
+  several models at once (all K right-hand sides of every model; one ADMM
+  product launch and one update launch per iteration for the whole NNLS
+  group, ``nnls.solve_group``, groups sized under an NNLS memory budget),
+  then per model the maps and the chi^2.
 
 Every model's value equals model_likelihood(theta_m, ...) on the first n_m
 orbits.
@@ -40,15 +44,36 @@ def _groups(n_orbits, row_bytes, budget):
     return groups
 
 
+def _nnls_groups(models, n_orbits, m_rows, K, budget):
+    """Consecutive runs of models whose NNLS solve states fit the budget
+    (at most NNLS_GROUP_MAX per run, at least one)."""
+    L = _lib.lib()
+    groups, cur, size = [], [], 0
+    for mi in models:
+        b = int(L.smx_nnls_group_state_bytes(int(n_orbits[mi]), int(m_rows), int(K)))
+        if cur and (size + b > budget or len(cur) == _lib.NNLS_GROUP_MAX):
+            groups.append(cur)
+            cur, size = [], 0
+        cur.append(mi)
+        size += b
+    if cur:
+        groups.append(cur)
+    return groups
+
+
 def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
                            sigma_amplify=1.0, schedule=None, nnls_maxiter=200, nnls_tol=0.0,
-                           library_budget_bytes=48 << 30, autotune=True, return_aux=False):
+                           library_budget_bytes=48 << 30, nnls_budget_bytes=80 << 30,
+                           grouped_nnls=True, autotune=True, return_aux=False):
     """logL for each row of thetas (M, 12); returns a CUDA tensor (M,).
 
     n_orbits: per-model orbit counts (default: all orbits for every model).
     schedule: one schedule dict, or a list of M (their N_max may differ).
     autotune: pick the ADMM product's occupancy for the batch's median shape
     first (``tune.ensure_tuned``, once per process and shape bucket).
+    grouped_nnls: solve the NNLS of several models in grouped launches (the
+    solve states of one NNLS group must fit nnls_budget_bytes); False, or
+    nnls_tol > 0, solves model by model.
     """
     thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
     M = thetas.shape[0]
@@ -58,11 +83,11 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
     n_orbits = [n_all] * M if n_orbits is None else [int(v) for v in n_orbits]
     if len(n_orbits) != M or min(n_orbits) < 1 or max(n_orbits) > n_all:
         raise ValueError("n_orbits must hold M counts in [1, number of orbits in dict_data]")
+    K_rhs = np.asarray(dict_data['XY_standard_normal']).shape[0]
     if autotune:
         from . import tune
         m_rows = int(Rzphi_n_tot) + 5 * int(num_Vbin)
-        K = np.asarray(dict_data['XY_standard_normal']).shape[0]
-        tune.ensure_tuned(int(np.median(n_orbits)), m_rows, K)
+        tune.ensure_tuned(int(np.median(n_orbits)), m_rows, K_rhs)
     scheds = schedule if isinstance(schedule, (list, tuple)) else [schedule] * M
     scheds = [MD.schedule_of(s) for s in scheds]
     r = int(scheds[0]['n_realizations'])
@@ -141,11 +166,24 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
         if gi + 1 < len(groups):
             pending = launch(groups[gi + 1])        # integrates while this group's NNLS runs
         main.wait_event(ev)
-        for m, o in zip(g, out):
-            md, info = MD.model_from_library(o['lib'], P, B, pbs[m], dict_data, nnls_maxiter, 1.0,
-                                             (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0)), nnls_tol)
-            logl[m] = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
-            if return_aux:
-                aux[m] = dict(lib=o['lib'], weights=md['weights'], nnls_iters=info['nnls_iters'])
-        del out
+        libs = dict(zip(g, (o['lib'] for o in out)))
+        cw = (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0))
+        if grouped_nnls and nnls_tol == 0.0:
+            for sub in _nnls_groups(g, n_orbits, P + 5 * B, K_rhs, nnls_budget_bytes):
+                res = MD.models_from_libraries([libs[m] for m in sub], P, B, [pbs[m] for m in sub],
+                                               dict_data, nnls_maxiter, 1.0, cw)
+                for m, (md, info) in zip(sub, res):
+                    logl[m] = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
+                    if return_aux:
+                        aux[m] = dict(lib=libs[m], weights=md['weights'],
+                                      nnls_iters=info['nnls_iters'])
+                del res
+        else:
+            for m in g:
+                md, info = MD.model_from_library(libs[m], P, B, pbs[m], dict_data, nnls_maxiter,
+                                                 1.0, cw, nnls_tol)
+                logl[m] = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
+                if return_aux:
+                    aux[m] = dict(lib=libs[m], weights=md['weights'], nnls_iters=info['nnls_iters'])
+        del out, libs
     return (logl, aux) if return_aux else logl

@@ -600,10 +600,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
 // One ADMM product: the full one on R (scales from rmax), or the delta one on
 // D (scales from dmax), as admm_full_product decides.  D == nullptr: always
 // full (force must then be 1).
+// The product of one workgroup: block `blk` of the launch's `nblk` blocks
+// (blockIdx.x and gridDim.x for one model's launch; a grouped launch passes
+// the block's index within its model's run of blocks).
 template <int KC, bool NT>
-__global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
-    int64_t K, int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
-    const unsigned* __restrict__ rmax, const float* __restrict__ D,
+__device__ __forceinline__ void admm_gemm_body(
+    int64_t blk, int64_t nblk, int64_t K, int64_t Kp, int64_t npad, int chunks_per_split,
+    const float* __restrict__ R, const unsigned* __restrict__ rmax, const float* __restrict__ D,
     const unsigned* __restrict__ dmax, const unsigned* __restrict__ fmax_, int force, float tau,
     const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml, float* __restrict__ part) {
   static_assert(KC == AG_KP, "the blocked layouts use 32-index chunks");
@@ -618,7 +621,7 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
   // XCD a contiguous run of this order, so an XCD's L2 holds the R (or D)
   // chunk ranges of one or two splits rather than all of R.
   const int64_t nrb = Kp / AG_KR, ncb = npad / AG_BN;
-  const int64_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t bid = xcd_block(blk, nblk);
   const int64_t k0 = (bid % nrb) * AG_KR;
   const int64_t j0 = ((bid / nrb) % ncb) * AG_BN;
   const int split = (int)(bid / (nrb * ncb));
@@ -653,6 +656,16 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
         const int row = rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
         if (row < rows) out[(int64_t)row * npad + cb * 32 + c] = acc[rb][cb][r];
       }
+}
+
+template <int KC, bool NT>
+__global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
+    int64_t K, int64_t Kp, int64_t npad, int chunks_per_split, const float* __restrict__ R,
+    const unsigned* __restrict__ rmax, const float* __restrict__ D,
+    const unsigned* __restrict__ dmax, const unsigned* __restrict__ fmax_, int force, float tau,
+    const _Float16* __restrict__ Mh, const _Float16* __restrict__ Ml, float* __restrict__ part) {
+  admm_gemm_body<KC, NT>(blockIdx.x, gridDim.x, K, Kp, npad, chunks_per_split, R, rmax, D, dmax,
+                         fmax_, force, tau, Mh, Ml, part);
 }
 
 __device__ __forceinline__ float block_max_256(float v, float* red);
@@ -785,20 +798,45 @@ struct AdmmIter {
   unsigned* full_count;     // full products taken
 };
 
+// Iteration `it` of a solve whose nine row-maximum sets start at rmax: the
+// product reads set it % 3 (of each kind), the update fills set (it + 1) % 3
+// for the next product and clears set (it + 2) % 3 for the one after.
+__host__ __device__ inline AdmmIter admm_iter(float* R, float* Rb, float* D, double* wb,
+                                              const double* Ra, unsigned* rmax,
+                                              unsigned* full_count, int64_t Kp, int it) {
+  AdmmIter ib;
+  ib.R = R;
+  ib.Rb = Rb;
+  ib.D = D;
+  ib.wb = wb;
+  ib.Ra = Ra;
+  ib.rcur = rmax + (it % 3) * Kp;
+  ib.rnext = rmax + ((it + 1) % 3) * Kp;
+  ib.rclear = rmax + ((it + 2) % 3) * Kp;
+  ib.dcur = rmax + (3 + it % 3) * Kp;
+  ib.dnext = rmax + (3 + (it + 1) % 3) * Kp;
+  ib.dclear = rmax + (3 + (it + 2) % 3) * Kp;
+  ib.fcur = rmax + (6 + it % 3) * Kp;
+  ib.fnext = rmax + (6 + (it + 1) % 3) * Kp;
+  ib.fclear = rmax + (6 + (it + 2) % 3) * Kp;
+  ib.full_count = full_count;
+  return ib;
+}
+
 // anchor == 1: this iteration's product was not run (the anchor was just
 // refreshed, so the operand R - R_a is zero): the partial sums count as 0 and
 // the iteration counts as a full product (new base R_b = 0, w_b = 0).
+// One block of the update: elements [256 V bx, 256 V (bx + 1)) of row k.
 template <int V>
-__global__ __launch_bounds__(256) void admm_update_kernel(
-    int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
-    const double* __restrict__ rho_p, const double* __restrict__ wc, double* __restrict__ x,
-    int first, int force, int anchor, float tau, double w_init, AdmmIter b, double alpha,
-    const int* __restrict__ active, unsigned* __restrict__ delta) {
+__device__ __forceinline__ void admm_update_body(
+    int64_t bx, int64_t k, int64_t K, int64_t n, int64_t Kp, int64_t npad, int S,
+    const float* __restrict__ part, const double* __restrict__ rho_p, const double* __restrict__ wc,
+    double* __restrict__ x, int first, int force, int anchor, float tau, double w_init,
+    const AdmmIter& b, double alpha, const int* __restrict__ active, unsigned* __restrict__ delta) {
   __shared__ float red[4];
   const bool full = admm_full_product(force, K, b.fcur, b.dcur, tau);
-  const int64_t k = blockIdx.y;
-  const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * V;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int64_t i0 = (bx * (int64_t)blockDim.x + threadIdx.x) * V;
+  if (bx == 0 && threadIdx.x == 0) {
     b.rclear[k] = 0u;
     b.dclear[k] = 0u;
     b.fclear[k] = 0u;
@@ -900,6 +938,79 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
   }
 }
 
+template <int V>
+__global__ __launch_bounds__(256) void admm_update_kernel(
+    int64_t K, int64_t n, int64_t Kp, int64_t npad, int S, const float* __restrict__ part,
+    const double* __restrict__ rho_p, const double* __restrict__ wc, double* __restrict__ x,
+    int first, int force, int anchor, float tau, double w_init, AdmmIter b, double alpha,
+    const int* __restrict__ active, unsigned* __restrict__ delta) {
+  admm_update_body<V>(blockIdx.x, blockIdx.y, K, n, Kp, npad, S, part, rho_p, wc, x, first, force,
+                      anchor, tau, w_init, b, alpha, active, delta);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped ADMM loop: the solves of several models (one K for all) advance one
+// iteration per launch pair, one product launch and one update launch for
+// the whole group.  Each model owns a contiguous run of a launch's blocks.
+// A product run starts at a multiple of 8, so that a block's index within
+// its run and its index in the launch agree modulo 8 and xcd_block gives
+// every model the XCD-contiguous order of its own launch: each model's
+// arithmetic is that of its single-model solve.
+// ---------------------------------------------------------------------------
+struct AdmmModel {
+  int64_t n, npad;
+  int64_t pblk0, pnblk;   // product blocks [pblk0, pblk0 + pnblk); the run is padded to 8
+  int64_t ublk0, unblk;   // update blocks (x) [ublk0, ublk0 + unblk)
+  int S, cps;
+  float* Rp;
+  float* Rb;
+  float* D;
+  double* wb;
+  const double* Ra;
+  unsigned* rmax;
+  unsigned* full_count;
+  const _Float16* Mh;
+  const _Float16* Ml;
+  float* part;
+  const double* rho64;
+  const double* Wc;
+  double* x;
+  const int* active;
+  unsigned* delta;
+  double w_init;
+};
+
+// The model whose run holds launch block b (runs are in increasing order).
+__device__ __forceinline__ int admm_model_of(const AdmmModel* __restrict__ mods, int G, int64_t b,
+                                             bool product) {
+  int g = 0;
+  for (int q = 1; q < G; ++q)
+    if (b >= (product ? mods[q].pblk0 : mods[q].ublk0)) g = q;
+  return g;
+}
+
+template <int KC, bool NT>
+__global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_group_kernel(
+    const AdmmModel* __restrict__ mods, int G, int64_t K, int64_t Kp, int it, int force, float tau) {
+  const AdmmModel& d = mods[admm_model_of(mods, G, blockIdx.x, true)];
+  const int64_t blk = (int64_t)blockIdx.x - d.pblk0;
+  if (blk >= d.pnblk) return;   // padding block: the whole workgroup leaves
+  const AdmmIter ib = admm_iter(d.Rp, d.Rb, d.D, d.wb, d.Ra, d.rmax, d.full_count, Kp, it);
+  admm_gemm_body<KC, NT>(blk, d.pnblk, K, Kp, d.npad, d.cps, d.Rp, ib.rcur, d.D, ib.dcur, ib.fcur,
+                         force, tau, d.Mh, d.Ml, d.part);
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void admm_update_group_kernel(
+    const AdmmModel* __restrict__ mods, int G, int64_t K, int64_t Kp, int it, int first, int force,
+    int anchor, float tau, double alpha) {
+  const AdmmModel& d = mods[admm_model_of(mods, G, blockIdx.x, false)];
+  const int64_t bx = (int64_t)blockIdx.x - d.ublk0;
+  const AdmmIter ib = admm_iter(d.Rp, d.Rb, d.D, d.wb, d.Ra, d.rmax, d.full_count, Kp, it);
+  admm_update_body<V>(bx, blockIdx.y, K, d.n, Kp, d.npad, d.S, d.part, d.rho64, d.Wc, d.x, first,
+                      force, anchor, tau, d.w_init, ib, alpha, d.active, d.delta);
+}
+
 // Anchor refresh (before the product of an iteration): R_a = rho (z - u) =
 // rho |x| of the current iterate (rho w_init before the first), V = R_a - c
 // for the FP64 product W_a = M V that follows, and the operand R - R_a = 0.
@@ -954,7 +1065,7 @@ struct NnlsWs {
                     // three of |D|, three of |R| = rho |x|
   int64_t npad, Kp;
   int S;
-  int64_t bytes;
+  int64_t scratch_bytes, state_bytes, bytes;
 };
 
 // Splits of the ADMM product's reduction: enough workgroups for `occ` on each
@@ -1006,10 +1117,17 @@ static int admm_splits(int64_t npad, int64_t Kp) {
   return admm_splits_for(npad, Kp, g_admm_occ.load(std::memory_order_relaxed));
 }
 
-static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
+// The workspace of a solve has two parts.  The factor scratch holds the
+// Gram's fp16 planes of U and the recursive inverse's work area, which the
+// FP64 GEMMs also use for their split-K partials; the solve state holds
+// everything the ADMM loop keeps.  A single solve keeps both in one buffer
+// (scratch first, `carve`); a grouped solve gives each model its own state
+// and lets the models share one scratch, sized for the largest (the models'
+// Gram, inverse and anchor products run one after another).
+static NnlsWs carve_parts(void* scratch, void* state, int64_t n, int64_t m, int64_t K) {
   NnlsWs w{};
   const int64_t ldu = align_up(m, 16);
-  char* p = static_cast<char*>(base);
+  char* p = static_cast<char*>(scratch);
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
     char* q = p ? p + off : nullptr;
@@ -1021,8 +1139,11 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.Uh = reinterpret_cast<_Float16*>(take(w.gpad * w.mpad * 2));
   w.Ul = reinterpret_cast<_Float16*>(take(w.gpad * w.mpad * 2));
   w.us = reinterpret_cast<float*>(take(w.gpad * 4));
-  w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
   w.work64 = reinterpret_cast<double*>(take((n * n + n) * 8));
+  w.scratch_bytes = off;
+  p = static_cast<char*>(state);
+  off = 0;
+  w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
   w.Ys = reinterpret_cast<double*>(take(K * ldu * 8));
   w.C = reinterpret_cast<double*>(take(K * n * 8));
   w.Wc = reinterpret_cast<double*>(take(K * n * 8));
@@ -1047,8 +1168,15 @@ static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
   w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.wb = reinterpret_cast<double*>(take(K * n * 8));
   w.full_count = reinterpret_cast<unsigned*>(take(4));
-  w.bytes = off;
+  w.state_bytes = off;
+  w.bytes = w.scratch_bytes + w.state_bytes;
   return w;
+}
+
+static NnlsWs carve(void* base, int64_t n, int64_t m, int64_t K) {
+  char* b = static_cast<char*>(base);
+  const int64_t sb = carve_parts(nullptr, nullptr, n, m, K).scratch_bytes;
+  return carve_parts(b, b ? b + sb : nullptr, n, m, K);
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
@@ -1583,22 +1711,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     const bool chk = tol > 0.0 && ((it + 1) % check == 0);
     if (chk)
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.delta, K, 0u);
-    AdmmIter ib;
-    ib.R = w.Rp;
-    ib.Rb = w.Rb;
-    ib.D = w.D;
-    ib.wb = w.wb;
-    ib.rcur = w.rmax + (it % 3) * Kp;
-    ib.rnext = w.rmax + ((it + 1) % 3) * Kp;
-    ib.rclear = w.rmax + ((it + 2) % 3) * Kp;
-    ib.dcur = w.rmax + (3 + it % 3) * Kp;
-    ib.dnext = w.rmax + (3 + (it + 1) % 3) * Kp;
-    ib.dclear = w.rmax + (3 + (it + 2) % 3) * Kp;
-    ib.fcur = w.rmax + (6 + it % 3) * Kp;
-    ib.fnext = w.rmax + (6 + (it + 1) % 3) * Kp;
-    ib.fclear = w.rmax + (6 + (it + 2) % 3) * Kp;
-    ib.Ra = w.Ra;
-    ib.full_count = w.full_count;
+    const AdmmIter ib = admm_iter(w.Rp, w.Rb, w.D, w.wb, w.Ra, w.rmax, w.full_count, Kp, it);
     // the first product has no base; the last is full so that the result
     // carries no delta error.  An anchor iteration replaces its product by
     // the FP64 one: the operand is zero and w = W_a exactly.
@@ -1653,6 +1766,178 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
                      done > 0 ? 1 : 0, w_init, W);
   SMX_LAUNCHED();
   if (iters_done) *iters_done = done;
+  return 0;
+}
+
+int64_t smx_nnls_group_state_bytes(int64_t n, int64_t m, int64_t K) {
+  if (n <= 0 || m <= 0 || K <= 0) return -1;
+  return carve_parts(nullptr, nullptr, n, m, K).state_bytes;
+}
+
+// The shared scratch of a grouped solve: the factor scratch of the largest
+// model, then the device copy of the models' descriptors.
+static int64_t group_desc_offset(int64_t n_max, int64_t m, int64_t K) {
+  return carve_parts(nullptr, nullptr, n_max, m, K).scratch_bytes;
+}
+
+int64_t smx_nnls_group_scratch_bytes(int64_t n_max, int64_t m, int64_t K) {
+  if (n_max <= 0 || m <= 0 || K <= 0) return -1;
+  return group_desc_offset(n_max, m, K) + align_up(SMX_NNLS_GROUP_MAX * (int64_t)sizeof(AdmmModel), 256);
+}
+
+int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t m, int64_t K,
+                        double lambda_reg, int32_t maxiter, void* scratch, void* stream) {
+  const int G = n_models;
+  SMX_REQUIRE(G >= 1 && G <= SMX_NNLS_GROUP_MAX, "smx_nnls_admm_group: n_models must be 1 .. %d",
+              SMX_NNLS_GROUP_MAX);
+  SMX_REQUIRE(probs && scratch, "smx_nnls_admm_group: null pointer");
+  SMX_REQUIRE(m > 0 && K > 0 && maxiter >= 0 && m < (int64_t)1 << 31,
+              "smx_nnls_admm_group: bad sizes");
+  int64_t n_max = 0;
+  for (int g = 0; g < G; ++g) {
+    const smx_nnls_problem& p = probs[g];
+    SMX_REQUIRE(p.lib && p.row_scale && p.Y && p.state && p.W,
+                "smx_nnls_admm_group: null pointer in problem %d", g);
+    SMX_REQUIRE(p.n > 0 && p.n < (int64_t)1 << 31 && p.ld_lib >= m,
+                "smx_nnls_admm_group: bad sizes in problem %d", g);
+    if (p.n > n_max) n_max = p.n;
+  }
+  hipStream_t st = as_stream(stream);
+  std::vector<NnlsWs> ws(G);
+  for (int g = 0; g < G; ++g) ws[g] = carve_parts(scratch, probs[g].state, probs[g].n, m, K);
+  const int64_t Kp = ws[0].Kp;
+  const int64_t ldu = align_up(m, 16);
+  // per model, one after another on the shared scratch: Gram, FP64 inverse,
+  // c = -Y U, the split planes of M, the loop's initial state
+  for (int g = 0; g < G; ++g) {
+    const smx_nnls_problem& p = probs[g];
+    const NnlsWs& w = ws[g];
+    const int64_t n = p.n, npad = w.npad;
+    stage_begin(ST_GRAM, st);
+    if (int e = form_gram64(st, w, p.lib, p.ld_lib, n, m, p.row_scale, lambda_reg)) return e;
+    stage_end(ST_GRAM, st, 3);
+    stage_begin(ST_FACTOR, st);
+    if (int e = spd_inverse64(st, w.Q64, n, n, w.work64)) return e;
+    stage_end(ST_FACTOR, st, 1);
+    stage_begin(ST_RHS, st);
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, p.Y, K,
+                       m, p.row_scale, ldu, w.Ys);
+    SMX_LAUNCHED();
+    if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, p.lib, p.ld_lib, 0.0, w.C, n, 0,
+                                w.work64, n * n + n))
+      return e;
+    hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad / 8, 256)), dim3(256), 0, st,
+                       w.Q64, n, npad, w.rho, w.Mh, w.Ml);
+    for (float* buf : {w.Rp, w.Rb, w.D})
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(Kp * npad, 256)), dim3(256), 0, st,
+                         reinterpret_cast<unsigned*>(buf), Kp * npad, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(9 * Kp, 256)), dim3(256), 0, st, w.rmax,
+                       9 * Kp, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, w.full_count, 1, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned*>(w.active), K, 1u);
+    SMX_LAUNCHED();
+    stage_end(ST_RHS, st, 4);
+  }
+  // the models' descriptors and block runs
+  bool v4 = true;
+  for (int g = 0; g < G; ++g) v4 = v4 && probs[g].n % 4 == 0;
+  const int V = v4 ? 4 : 1;
+  std::vector<AdmmModel> md(G);
+  int64_t pb = 0, ub = 0;
+  for (int g = 0; g < G; ++g) {
+    const NnlsWs& w = ws[g];
+    AdmmModel& d = md[g];
+    d.n = probs[g].n;
+    d.npad = w.npad;
+    d.S = w.S;
+    d.cps = static_cast<int>((w.npad / AG_KP + w.S - 1) / w.S);
+    d.pblk0 = pb;
+    d.pnblk = (w.npad / AG_BN) * w.S * (Kp / AG_KR);
+    d.ublk0 = ub;
+    d.unblk = blocks_for(d.n / V, 256);
+    pb += align_up(d.pnblk, 8);
+    ub += d.unblk;
+    d.Rp = w.Rp;
+    d.Rb = w.Rb;
+    d.D = w.D;
+    d.wb = w.wb;
+    d.Ra = w.Ra;
+    d.rmax = w.rmax;
+    d.full_count = w.full_count;
+    d.Mh = w.Mh;
+    d.Ml = w.Ml;
+    d.part = w.part;
+    d.rho64 = w.rho64;
+    d.Wc = w.Wc;
+    d.x = w.x;
+    d.active = w.active;
+    d.delta = w.delta;
+    d.w_init = probs[g].w_init;
+  }
+  SMX_REQUIRE(pb < ((int64_t)1 << 31) && ub < ((int64_t)1 << 31),
+              "smx_nnls_admm_group: too many blocks for one launch");
+  AdmmModel* dmods =
+      reinterpret_cast<AdmmModel*>(static_cast<char*>(scratch) + group_desc_offset(n_max, m, K));
+  SMX_HIP(hipMemcpyAsync(dmods, md.data(), G * sizeof(AdmmModel), hipMemcpyHostToDevice, st));
+  SMX_HIP(hipStreamSynchronize(st));   // md is pageable host memory
+  stage_begin(ST_ADMM, st);
+  for (int it = 0; it < maxiter; ++it) {
+    const int anchor = (it % AG_ANCHOR == 0) ? 1 : 0;
+    const int force = (anchor || it == maxiter - 1) ? 1 : 0;
+    const int first = it == 0 ? 1 : 0;
+    const float tau = g_admm_tau.load(std::memory_order_relaxed);
+    if (anchor) {
+      for (int g = 0; g < G; ++g) {
+        const NnlsWs& w = ws[g];
+        const int64_t n = probs[g].n;
+        hipLaunchKernelGGL(admm_anchor_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0,
+                           st, K, n, Kp, first, probs[g].w_init, w.rho64, w.x, w.C, w.Ra, w.Va,
+                           w.Rp);
+        SMX_LAUNCHED();
+        if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n, 0,
+                                    w.work64, n * n + n))
+          return e;
+      }
+    } else {
+      stage_begin(ST_ADMM_PRODUCT, st);
+      if (Kp == AG_KR)
+        hipLaunchKernelGGL((admm_gemm_group_kernel<AG_KP, true>), dim3((unsigned)pb), dim3(AG_NT), 0,
+                           st, dmods, G, K, Kp, it, force, tau);
+      else
+        hipLaunchKernelGGL((admm_gemm_group_kernel<AG_KP, false>), dim3((unsigned)pb), dim3(AG_NT),
+                           0, st, dmods, G, K, Kp, it, force, tau);
+      SMX_LAUNCHED();
+      stage_end(ST_ADMM_PRODUCT, st, 1);
+    }
+    if (V == 4)
+      hipLaunchKernelGGL(admm_update_group_kernel<4>, dim3((unsigned)ub, (unsigned)K), dim3(256), 0,
+                         st, dmods, G, K, Kp, it, first, force, anchor, tau, 1.6);
+    else
+      hipLaunchKernelGGL(admm_update_group_kernel<1>, dim3((unsigned)ub, (unsigned)K), dim3(256), 0,
+                         st, dmods, G, K, Kp, it, first, force, anchor, tau, 1.6);
+    SMX_LAUNCHED();
+  }
+  stage_end(ST_ADMM, st, maxiter);
+  for (int g = 0; g < G; ++g) {
+    const int64_t tot = K * probs[g].n;
+    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, ws[g].x,
+                       maxiter > 0 ? 1 : 0, probs[g].w_init, probs[g].W);
+  }
+  SMX_LAUNCHED();
+  return 0;
+}
+
+int smx_nnls_group_full_products(int64_t n, int64_t m, int64_t K, const void* state,
+                                 int32_t* full_products, void* stream) {
+  SMX_REQUIRE(state && full_products && n > 0 && m > 0 && K > 0,
+              "smx_nnls_group_full_products: bad arguments");
+  hipStream_t st = as_stream(stream);
+  NnlsWs w = carve_parts(nullptr, const_cast<void*>(state), n, m, K);
+  unsigned v = 0;
+  SMX_HIP(hipMemcpyAsync(&v, w.full_count, 4, hipMemcpyDeviceToHost, st));
+  SMX_HIP(hipStreamSynchronize(st));
+  *full_products = static_cast<int32_t>(v);
   return 0;
 }
 

@@ -248,29 +248,55 @@ def prepare_targets(n, params_disk_rho, dict_data):
                 mmpo=mmpo, gamma_kin=gamma_kin, s_ap=s_ap, v0_ap=v0_ap)
 
 
+def _bootstrap_data(n, params_disk_rho, dict_data):
+    """Targets and the bootstrapped observations (model.py:192-258)."""
+    t = prepare_targets(n, params_disk_rho, dict_data)
+    D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
+    y_xy, sig_xy, y_h, sig_h = t['y_xy'], t['sig_xy'], t['y_h'], t['sig_h']
+    t['y_xy_boot'] = y_xy[None, :] + D('XY_standard_normal') * sig_xy[None, :]
+    t['y_h_boot'] = [y_h[k][None, :] + D('h%d_standard_normal' % (k + 1)) * sig_h[k][None, :]
+                     for k in range(4)]
+    return t
+
+
+def _model_maps(W, lib, P, B, t):
+    """The model dict of model.py:260-286 from the weights."""
+    d3, d2, h1, h2, h3, h4, V, sig = compute_model_single(W, lib, P, B, t['gamma_kin'], t['v0_ap'],
+                                                          t['s_ap'])
+    d2 = d2 * t['mmpo'] * t['lm']
+    return {'density_3d': t['y_R'], 'density_3d_model': d3, 'surface_density': d2, 'h1': h1,
+            'h2': h2, 'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
+
+
 def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regularization,
                        channel_w, nnls_tol=0.0):
     """Everything build_model does after the library (model.py:192-286):
     density target, data normalisation, bootstrap NNLS and the model maps."""
     from .nnls import solve_nnls_admm_bootstrap_lib
     n = lib.shape[0]
-    t = prepare_targets(n, params_disk_rho, dict_data)
-    lm, y_R, sig_R, y_xy, sig_xy = t['lm'], t['y_R'], t['sig_R'], t['y_xy'], t['sig_xy']
-    y_h, sig_h, mmpo, gamma_kin = t['y_h'], t['sig_h'], t['mmpo'], t['gamma_kin']
-    s_ap, v0_ap = t['s_ap'], t['v0_ap']
-    D = lambda k: _lib.dev(dict_data[k])  # noqa: E731
-    y_xy_boot = y_xy[None, :] + D('XY_standard_normal') * sig_xy[None, :]
-    y_h_boot = [y_h[k][None, :] + D('h%d_standard_normal' % (k + 1)) * sig_h[k][None, :]
-                for k in range(4)]
-    W, info = solve_nnls_admm_bootstrap_lib(lib, P, B, y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R,
-                                            sig_xy, sig_h, lambda_reg=regularization,
-                                            maxiter=nnls_maxiter, w_rzphi=channel_w[0],
-                                            w_xy=channel_w[1], w_h=channel_w[2], tol=nnls_tol)
-    d3, d2, h1, h2, h3, h4, V, sig = compute_model_single(W, lib, P, B, gamma_kin, v0_ap, s_ap)
-    d2 = d2 * mmpo * lm
-    out = {'density_3d': y_R, 'density_3d_model': d3, 'surface_density': d2, 'h1': h1, 'h2': h2,
-           'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
-    return out, dict(mmpo=mmpo, n=n, nnls_iters=info['iters'])
+    t = _bootstrap_data(n, params_disk_rho, dict_data)
+    W, info = solve_nnls_admm_bootstrap_lib(lib, P, B, t['y_R'], t['gamma_kin'], t['y_xy_boot'],
+                                            t['y_h_boot'], t['sig_R'], t['sig_xy'], t['sig_h'],
+                                            lambda_reg=regularization, maxiter=nnls_maxiter,
+                                            w_rzphi=channel_w[0], w_xy=channel_w[1],
+                                            w_h=channel_w[2], tol=nnls_tol)
+    return _model_maps(W, lib, P, B, t), dict(mmpo=t['mmpo'], n=n, nnls_iters=info['iters'])
+
+
+def models_from_libraries(libs, P, B, params_disk_rhos, dict_data, nnls_maxiter, regularization,
+                          channel_w):
+    """model_from_library for several models (one galaxy, the same P and B) with
+    their NNLS solves grouped: one ADMM product launch and one update launch
+    per iteration for all of them (``nnls.solve_group``).  Each model's result
+    equals its model_from_library call (nnls_tol = 0)."""
+    from .nnls import solve_group
+    ts = [_bootstrap_data(lib.shape[0], pb, dict_data) for lib, pb in zip(libs, params_disk_rhos)]
+    Ws = solve_group(libs, P, B, [(t['y_R'], t['gamma_kin'], t['y_xy_boot'], t['y_h_boot'],
+                                   t['sig_R'], t['sig_xy'], t['sig_h']) for t in ts],
+                     regularization, nnls_maxiter, *channel_w)
+    return [(_model_maps(W, lib, P, B, t), dict(mmpo=t['mmpo'], n=lib.shape[0],
+                                                nnls_iters=int(nnls_maxiter)))
+            for W, lib, t in zip(Ws, libs, ts)]
 
 
 def build_model(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,

@@ -83,6 +83,65 @@ def solve_nnls_admm_bootstrap_lib(lib, P, B, y_R, gamma_kin, y_xy_boot, y_h_boot
     return W, {'iters': it}
 
 
+def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy=1.0, w_h=1.0):
+    """Bootstrap solves (nnls.py:68-166) of several models in grouped launches
+    (``smx_nnls_admm_group``: per ADMM iteration, one product launch and one
+    update launch for all models).  libs[g] (n_g, >= P+5B) are the models'
+    library columns; data[g] = (y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R,
+    sig_xy, sig_h) as for solve_nnls_admm_bootstrap_lib, with one K for all.
+    Returns each model's weights (K, n_g), equal to its single solve (tol 0)."""
+    L = _lib.lib()
+    if len(libs) != len(data):
+        raise ValueError("solve_group: one data tuple per library")
+    if len(libs) > _lib.NNLS_GROUP_MAX:
+        h = _lib.NNLS_GROUP_MAX
+        return (solve_group(libs[:h], P, B, data[:h], lambda_reg, maxiter, w_rzphi, w_xy, w_h) +
+                solve_group(libs[h:], P, B, data[h:], lambda_reg, maxiter, w_rzphi, w_xy, w_h))
+    G = len(libs)
+    if G == 0:
+        return []
+    m = P + 5 * B
+    f64 = torch.float64
+    probs = (_lib.SmxNnlsProblem * G)()
+    keep, Ws, K = [], [], None
+    for g, (lib, d) in enumerate(zip(libs, data)):
+        y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R, sig_xy, sig_h = d
+        row_scale, Y = _scales_and_rhs(P, B, y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R, sig_xy,
+                                       sig_h, w_rzphi, w_xy, w_h)
+        row_scale = row_scale.contiguous()
+        n = lib.shape[0]
+        if K is None:
+            K = Y.shape[0]
+        elif Y.shape[0] != K:
+            raise ValueError("solve_group: every model needs the same number of right-hand sides")
+        if lib.shape[1] < m:
+            raise ValueError("solve_group: library rows shorter than P + 5B")
+        state = torch.empty(int(L.smx_nnls_group_state_bytes(n, m, K)), dtype=torch.uint8,
+                            device='cuda')
+        W = torch.empty((K, n), dtype=f64, device='cuda')
+        probs[g] = _lib.SmxNnlsProblem(_lib.ptr(lib, f64).value, lib.shape[1], n,
+                                       _lib.ptr(row_scale, f64).value, _lib.ptr(Y, f64).value,
+                                       float(torch.sum(y_R).item()) / n, state.data_ptr(),
+                                       W.data_ptr())
+        keep += [row_scale, Y, state]
+        Ws.append(W)
+    n_max = max(lib.shape[0] for lib in libs)
+    scratch = torch.empty(int(L.smx_nnls_group_scratch_bytes(n_max, m, K)), dtype=torch.uint8,
+                          device='cuda')
+    _lib.check(L.smx_nnls_admm_group(G, probs, m, K, float(lambda_reg), int(maxiter),
+                                     _lib.ptr(scratch), _lib.stream_ptr()), 'smx_nnls_admm_group')
+    if _lib.TIMING['on']:
+        for g, lib in enumerate(libs):
+            full = ctypes.c_int32(0)
+            _lib.check(L.smx_nnls_group_full_products(lib.shape[0], m, K, ctypes.c_void_p(probs[g].state),
+                                                      ctypes.byref(full), _lib.stream_ptr()),
+                       'smx_nnls_group_full_products')
+            STATS['products'] += int(maxiter)
+            STATS['full_products'] += int(full.value)
+    del keep, scratch
+    return Ws
+
+
 def _stack_library(A_Rzphi, A_xy, A_h):
     A = [_lib.dev(a) for a in [A_Rzphi, A_xy] + list(A_h)]
     return torch.cat(A, dim=0).t().contiguous()      # (n, P + 5B)

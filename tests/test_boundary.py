@@ -29,6 +29,26 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.SmxPotential) == 8 * (5 + 3 + 9 + 4 + 3 + 9 + 5 + 5 + 1)
     assert ctypes.sizeof(_lib.SmxGrids) == 4 * 8 + 8 * (3 + 3 + 2 + 2 + 9 + 1)
     assert ctypes.sizeof(_lib.SmxSchedule) == 8 + 8 * 5
+    assert ctypes.sizeof(_lib.SmxNnlsProblem) == 8 * 8
+    assert _lib.SmxNnlsProblem.w_init.offset == 5 * 8
+
+
+def test_grouped_nnls_refuses_bad_arguments_without_gpu():
+    """smx_nnls_admm_group checks its arguments before touching the device."""
+    from schwarmax_amd import _lib
+    L = _lib.load_library()
+    probs = (_lib.SmxNnlsProblem * 1)()            # null buffers
+    fake = ctypes.c_void_p(256)                     # never dereferenced: refused first
+    assert L.smx_nnls_admm_group(1, probs, 100, 4, 1.0, 10, fake, None) == 2
+    assert 'null pointer in problem 0' in _lib.last_error()
+    assert L.smx_nnls_admm_group(0, probs, 100, 4, 1.0, 10, fake, None) == 2
+    assert L.smx_nnls_admm_group(_lib.NNLS_GROUP_MAX + 1, probs, 100, 4, 1.0, 10, fake, None) == 2
+    assert L.smx_nnls_group_state_bytes(0, 1, 1) == -1
+    assert L.smx_nnls_group_scratch_bytes(1000, 700, 20) > 8 * 1000 * 1000
+    # a model's state plus the shared scratch is the single solve's workspace
+    # (and the scratch's room for the models' descriptors)
+    assert (L.smx_nnls_group_state_bytes(1000, 700, 20) + L.smx_nnls_group_scratch_bytes(1000, 700, 20)
+            >= L.smx_nnls_workspace_bytes(1000, 700, 20))
 
 
 def test_workspace_queries_without_gpu():

@@ -78,3 +78,13 @@ def test_batch_groups_and_budget(setup):
                                      library_budget_bytes=1).cpu().numpy()
     assert np.all(np.isfinite(a))
     assert np.array_equal(a, b)
+    # the NNLS of the three models in one grouped solve, one per group, or
+    # model by model: the same values
+    c = batch.model_likelihood_batch(thetas, dd, NAP, dd['Rzphi_n_tot'], n_orbits=[40, 25, 33],
+                                     schedule=scheds, nnls_maxiter=50,
+                                     nnls_budget_bytes=1).cpu().numpy()
+    d = batch.model_likelihood_batch(thetas, dd, NAP, dd['Rzphi_n_tot'], n_orbits=[40, 25, 33],
+                                     schedule=scheds, nnls_maxiter=50,
+                                     grouped_nnls=False).cpu().numpy()
+    assert np.array_equal(a, c)
+    assert np.array_equal(a, d)

@@ -333,6 +333,28 @@ int smx_nnls_group_full_products(int64_t n, int64_t m, int64_t K, const void* st
  * started afterwards (workspaces are sized for the largest setting). */
 int smx_admm_set_occupancy(int32_t workgroups_per_cu);
 int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
+/* Launch-shape knobs, process-wide and atomic, read by each launch (value 0
+ * restores a knob's default).  None changes what a kernel computes; the
+ * split-K knobs change the order in which partial sums are added.
+ *   SMX_KNOB_ADMM_OCCUPANCY       1..8, default 2: as smx_admm_set_occupancy.
+ *   SMX_KNOB_GRAM_WAVES           4 or 8, default 4: waves per Gram workgroup
+ *                                 (128 x 64 or 128 x 32 outputs per wave).
+ *   SMX_KNOB_INV_SYRKX_MIN        >= 256, default 16384: nodes of the FP64
+ *                                 inverse with at least this many rows form
+ *                                 their symmetric updates with rocBLAS syrkx,
+ *                                 smaller ones with full dgemm products.
+ *   SMX_KNOB_GEMM64_SPLIT_TARGET  256..4096, default 2048: workgroups the
+ *                                 FP64 GEMM's split-K rule fills (right-hand
+ *                                 sides c = -Y U, the ADMM anchors). */
+enum {
+  SMX_KNOB_ADMM_OCCUPANCY = 0,
+  SMX_KNOB_GRAM_WAVES = 1,
+  SMX_KNOB_INV_SYRKX_MIN = 2,
+  SMX_KNOB_GEMM64_SPLIT_TARGET = 3,
+  SMX_KNOB_COUNT = 4
+};
+int smx_tune_set(int32_t knob, int64_t value);
+int smx_tune_get(int32_t knob, int64_t* value);
 /* Delta-product bound tau in [0, 0.25] (default 2^-10): an ADMM iteration
  * multiplies only R - R_b by the inverse's fp16 hi plane while every row has
  * max|R_k - R_b,k| <= tau max|R_k|, and takes the full split product (and a

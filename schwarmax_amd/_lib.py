@@ -130,6 +130,8 @@ _SIGS = [
     ('smx_nnls_admm', ctypes.c_int,
      [_P, _I64, _I64, _I64, _P, _P, _I64, _D, _I32, _D, _D, _P, _P, ctypes.POINTER(_I32), _P]),
     ('smx_nnls_admm_full_products', ctypes.c_int, [_I64, _I64, _I64, _P, ctypes.POINTER(_I32), _P]),
+    ('smx_tune_set', ctypes.c_int, [_I32, _I64]),
+    ('smx_tune_get', ctypes.c_int, [_I32, ctypes.POINTER(_I64)]),
     ('smx_nnls_group_state_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_group_scratch_bytes', _I64, [_I64, _I64, _I64]),
     ('smx_nnls_admm_group', ctypes.c_int,

@@ -63,17 +63,25 @@ def _nnls_groups(models, n_orbits, m_rows, K, budget):
 
 def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=None,
                            sigma_amplify=1.0, schedule=None, nnls_maxiter=200, nnls_tol=0.0,
-                           library_budget_bytes=48 << 30, nnls_budget_bytes=80 << 30,
+                           library_budget_bytes=48 << 30, nnls_budget_bytes=16 << 30,
                            grouped_nnls=True, autotune=True, return_aux=False):
     """logL for each row of thetas (M, 12); returns a CUDA tensor (M,).
 
     n_orbits: per-model orbit counts (default: all orbits for every model).
     schedule: one schedule dict, or a list of M (their N_max may differ).
-    autotune: pick the ADMM product's occupancy for the batch's median shape
-    first (``tune.ensure_tuned``, once per process and shape bucket).
+    autotune: pick the NNLS stages' launch-shape knobs (ADMM product occupancy,
+    Gram waves, inverse syrkx threshold, FP64 GEMM split target) for the
+    batch's median shape first (``tune.ensure_tuned``, once per process and
+    shape bucket).
     grouped_nnls: solve the NNLS of several models in grouped launches (the
     solve states of one NNLS group must fit nnls_budget_bytes); False, or
-    nnls_tol > 0, solves model by model.
+    nnls_tol > 0, solves model by model.  The default budget keeps groups
+    small: a grouped product launch of many models spans several rounds of
+    workgroups and keeps every SIMD busy, so the next library group's
+    persistent integration (one full-register wave per SIMD, on the side
+    stream) finds no free SIMD and stops overlapping the NNLS (20 models of
+    n ~ U[5k, 40k]: 23.3 s with 80 GB groups, 17.6 s with 40 GB groups or
+    model by model).
     """
     thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
     M = thetas.shape[0]
@@ -160,6 +168,14 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
 
     logl = torch.empty(M, dtype=torch.float64, device='cuda')
     aux = {}
+    arena, nnls_subs = None, []
+    if grouped_nnls and nnls_tol == 0.0:
+        # one arena for every NNLS group of the batch, sized for the largest
+        from .nnls import group_bytes
+        nnls_subs = [_nnls_groups(g, n_orbits, P + 5 * B, K_rhs, nnls_budget_bytes) for g in groups]
+        need = max(group_bytes([n_orbits[m] for m in sub], P + 5 * B, K_rhs)
+                   for subs in nnls_subs for sub in subs)
+        arena = torch.empty(need, dtype=torch.uint8, device='cuda')
     pending = launch(groups[0])
     for gi, g in enumerate(groups):
         out, ev = pending
@@ -169,9 +185,9 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
         libs = dict(zip(g, (o['lib'] for o in out)))
         cw = (np.sqrt(5.0), np.sqrt(5.0), np.sqrt(1.0))
         if grouped_nnls and nnls_tol == 0.0:
-            for sub in _nnls_groups(g, n_orbits, P + 5 * B, K_rhs, nnls_budget_bytes):
+            for sub in nnls_subs[gi]:
                 res = MD.models_from_libraries([libs[m] for m in sub], P, B, [pbs[m] for m in sub],
-                                               dict_data, nnls_maxiter, 1.0, cw)
+                                               dict_data, nnls_maxiter, 1.0, cw, arena=arena)
                 for m, (md, info) in zip(sub, res):
                     logl[m] = bootstrap_loglike(md, dict_data, sigma_amplify)[0]
                     if return_aux:
@@ -186,4 +202,5 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
                 if return_aux:
                     aux[m] = dict(lib=libs[m], weights=md['weights'], nnls_iters=info['nnls_iters'])
         del out, libs
+    del arena
     return (logl, aux) if return_aux else logl

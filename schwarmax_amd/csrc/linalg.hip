@@ -35,6 +35,7 @@
 
 #include "common.hpp"
 #include "timing.hpp"
+#include "tuning.hpp"
 #include "../../include/smx.h"
 
 namespace smx {
@@ -243,9 +244,10 @@ static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, con
   dim3 grid((unsigned)((N + NT_TN - 1) / NT_TN), (unsigned)((M + NT_TM - 1) / NT_TM));
   int64_t S = 1, kchunk = 0;
   const int64_t tiles = (int64_t)grid.x * grid.y;
-  if (work && !lower_only && tiles < 1024) {
-    S = (1024 + tiles - 1) / tiles;
-    if (S > 8) S = 8;
+  const int64_t target = knob(KN_GEMM64_SPLIT);   // workgroups the splits should fill
+  if (work && !lower_only && tiles < target) {
+    S = (target + tiles - 1) / tiles;
+    if (S > 16) S = 16;
     kchunk = ((K + S - 1) / S + NT_TK - 1) / NT_TK * NT_TK;
     S = (K + kchunk - 1) / kchunk;
     if (S < 2 || S * M * N > work_elems) S = 1;
@@ -1076,7 +1078,6 @@ struct NnlsWs {
 // partial buffer for occ = AG_OCC_MAX, and S grows with occ, so a later
 // setting never overruns one.
 constexpr int AG_OCC_MAX = 8;
-static std::atomic<int> g_admm_occ{AG_OCC};
 
 static int admm_splits_for(int64_t npad, int64_t Kp, int occ) {
   int64_t tiles = (npad / AG_BN) * (Kp / AG_KR);   // workgroups of one split
@@ -1114,7 +1115,7 @@ static void launch_admm_product(hipStream_t st, int64_t K, int64_t Kp, int64_t n
 static std::atomic<float> g_admm_tau{AG_TAU};
 
 static int admm_splits(int64_t npad, int64_t Kp) {
-  return admm_splits_for(npad, Kp, g_admm_occ.load(std::memory_order_relaxed));
+  return admm_splits_for(npad, Kp, static_cast<int>(knob(KN_ADMM_OCC)));
 }
 
 // The workspace of a solve has two parts.  The factor scratch holds the
@@ -1196,15 +1197,18 @@ __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
 // chunk c) = rows 128 I + r, indices 32 c + q at ((I nchk + c) 128 + r) 32 + q.
 // ---------------------------------------------------------------------------
 constexpr int GR_TM = 128;   // rows (A) per tile; the B operand is two such blocks
-// Waves per Gram workgroup.  Unlike the ADMM product, the Gram does three
-// MFMAs per fragment pair and every wave reads all 128 A rows from LDS: 8
-// waves of 128 x 32 outputs took 28.4 ms at plain-20k against 18.1 ms for 4
-// waves of 128 x 64.
-constexpr int GR_WAVES = 4;
-constexpr int GR_NT = 64 * GR_WAVES;
-constexpr int GR_WCB = 2 * GR_TM / (32 * GR_WAVES);     // 32-column blocks per wave
-constexpr int GR_PASSES = GR_TM / (GR_NT / 4);          // staging passes per 128-row tile
-static_assert(GR_WCB >= 1 && GR_PASSES >= 1, "Gram wave layout");
+// Waves per Gram workgroup (the tuning knob KN_GRAM_WAVES: 4 or 8).  Unlike
+// the ADMM product, the Gram does three MFMAs per fragment pair and every
+// wave reads all 128 A rows from LDS: 8 waves of 128 x 32 outputs took
+// 28.4 ms at plain-20k against 18.1 ms for 4 waves of 128 x 64.
+template <int WAVES>
+struct GramShape {
+  static constexpr int NT = 64 * WAVES;
+  static constexpr int WCB = 2 * GR_TM / (32 * WAVES);   // 32-column blocks per wave
+  static constexpr int PASSES = GR_TM / (NT / 4);         // staging passes per 128-row tile
+  static constexpr int OCC = WAVES == 4 ? AG_OCC : 1;     // workgroups per CU (registers)
+  static_assert(WCB >= 1 && PASSES >= 1, "Gram wave layout");
+};
 
 __device__ __forceinline__ int64_t g_tile_index(int64_t i, int64_t k, int64_t nchk) {
   return ((i / GR_TM * nchk + k / AG_KP) * GR_TM + i % GR_TM) * AG_KP + k % AG_KP;
@@ -1282,13 +1286,19 @@ static int64_t gram_grid(int64_t gpad) { return 8 * ((gram_tiles(gpad) + 7) / 8)
 #define GR_FLUSH 8
 #endif
 
-// One (128 x 256) tile of Q per workgroup (4 waves of 128 x 64), 32-index
-// chunks ch0 .. ch1 - 1 staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep
-// step; stores (first) or adds the segment's sums to Q[i][j] for j <= i < n.
-__global__ __launch_bounds__(GR_NT, AG_OCC) void gram_split_kernel(
+// One (128 x 256) tile of Q per workgroup (WAVES waves of 128 x 256/WAVES),
+// 32-index chunks ch0 .. ch1 - 1 staged through LDS, hi*hi + hi*lo + lo*hi
+// per 16-deep step; stores (first) or adds the segment's sums to Q[i][j] for
+// j <= i < n.
+template <int WAVES>
+__global__ __launch_bounds__(GramShape<WAVES>::NT, GramShape<WAVES>::OCC) void gram_split_kernel(
     int64_t n, int64_t gpad, int64_t mpad, const _Float16* __restrict__ Uh,
     const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q,
     int64_t ch0, int64_t ch1, int first) {
+  constexpr int GR_NT = GramShape<WAVES>::NT;
+  constexpr int GR_WCB = GramShape<WAVES>::WCB;
+  constexpr int GR_PASSES = GramShape<WAVES>::PASSES;
+  constexpr int GR_WAVES = WAVES;
   constexpr int P = AG_KP + 8;
   __shared__ __align__(16) _Float16 Bh[2 * GR_TM][P];
   __shared__ __align__(16) _Float16 Bl[2 * GR_TM][P];
@@ -1552,7 +1562,8 @@ static int blas_dsyrkx(hipStream_t st, bool trans, int64_t n, int64_t k, double 
 //   S  -= T A21^T   (lower)            dsyrkx(T): (A21^T)^T T^T ... = A21 T^T
 //   A21^T = -T^T S^-1                  dgemm(N, N, n1, n2, n2)
 //   X  -= T^T A21   (lower)            dsyrkx(N): A21^T (T^T)^T = A21^T T
-// Below INV_SYRKX_MIN rows the two symmetric updates run as full dgemm
+// Below knob(KN_INV_SYRKX_MIN) rows (default 16,384) the two symmetric
+// updates run as full dgemm
 // products instead: twice the flops, but one launch where rocBLAS syrkx runs
 // a diagonal-block kernel plus several off-diagonal products.  Measured on
 // the 20,000-row inverse, the full products are as fast as syrkx at the
@@ -1560,9 +1571,6 @@ static int blas_dsyrkx(hipStream_t st, bool trans, int64_t n, int64_t k, double 
 // syrkx keeps the 20,000-row node, where its halved flops win.  The full
 // product also rewrites the block's upper triangle, which nothing reads
 // before mirror_lower restores it.
-#ifndef INV_SYRKX_MIN
-#define INV_SYRKX_MIN 16384
-#endif
 static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double* work) {
   if (n <= INV_BASE) {
     hipLaunchKernelGGL(inv_base_kernel, dim3(1), dim3(1024), 0, st, A, lda, (int)n);
@@ -1578,7 +1586,7 @@ static int inv_spd_rec(hipStream_t st, double* A, int64_t n, int64_t lda, double
   double* deeper = work + n1 * n2;
   if (int e = inv_spd_rec(st, A11, n1, lda, deeper)) return e;
   mirror_lower(st, A11, n1, lda);
-  const bool sym = n >= INV_SYRKX_MIN;
+  const bool sym = n >= knob(KN_INV_SYRKX_MIN);
   if (int e = blas_dgemm(st, false, false, n1, n2, n1, 1.0, A11, lda, A21, lda, 0.0, T, n1)) return e;
   if (int e = sym ? blas_dsyrkx(st, true, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda)
                   : blas_dgemm(st, true, false, n2, n2, n1, -1.0, A21, lda, T, n1, 1.0, A22, lda))
@@ -1639,8 +1647,14 @@ static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64
   const int64_t seg = (nchk + GR_FLUSH - 1) / GR_FLUSH;
   for (int64_t ch0 = 0; ch0 < nchk; ch0 += seg) {
     const int64_t ch1 = ch0 + seg < nchk ? ch0 + seg : nchk;
-    hipLaunchKernelGGL(gram_split_kernel, dim3((unsigned)gram_grid(w.gpad)), dim3(GR_NT), 0, st, n,
-                       w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64, ch0, ch1, ch0 == 0 ? 1 : 0);
+    if (knob(KN_GRAM_WAVES) == 8)
+      hipLaunchKernelGGL(gram_split_kernel<8>, dim3((unsigned)gram_grid(w.gpad)),
+                         dim3(GramShape<8>::NT), 0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64,
+                         ch0, ch1, ch0 == 0 ? 1 : 0);
+    else
+      hipLaunchKernelGGL(gram_split_kernel<4>, dim3((unsigned)gram_grid(w.gpad)),
+                         dim3(GramShape<4>::NT), 0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64,
+                         ch0, ch1, ch0 == 0 ? 1 : 0);
   }
   SMX_LAUNCHED();
   hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
@@ -1843,11 +1857,19 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
   bool v4 = true;
   for (int g = 0; g < G; ++g) v4 = v4 && probs[g].n % 4 == 0;
   const int V = v4 ? 4 : 1;
+  // Product runs longest workgroups first (chunks per split, descending):
+  // the launch's last round then holds the short workgroups of small models
+  // rather than a tail of long ones.  The descriptors are stored in run order.
+  std::vector<int> order(G);
+  for (int g = 0; g < G; ++g) order[g] = g;
+  auto cps_of = [&](int g) { return (ws[g].npad / AG_KP + ws[g].S - 1) / ws[g].S; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cps_of(a) > cps_of(b); });
   std::vector<AdmmModel> md(G);
   int64_t pb = 0, ub = 0;
-  for (int g = 0; g < G; ++g) {
+  for (int q = 0; q < G; ++q) {
+    const int g = order[q];
     const NnlsWs& w = ws[g];
-    AdmmModel& d = md[g];
+    AdmmModel& d = md[q];
     d.n = probs[g].n;
     d.npad = w.npad;
     d.S = w.S;
@@ -2012,8 +2034,7 @@ int smx_nnls_admm_full_products(int64_t n, int64_t m, int64_t K, const void* wor
 int smx_admm_set_occupancy(int32_t workgroups_per_cu) {
   SMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= AG_OCC_MAX,
               "smx_admm_set_occupancy: workgroups_per_cu must be 0 (default) .. %d", AG_OCC_MAX);
-  g_admm_occ.store(workgroups_per_cu == 0 ? AG_OCC : workgroups_per_cu, std::memory_order_relaxed);
-  return 0;
+  return smx_tune_set(SMX_KNOB_ADMM_OCCUPANCY, workgroups_per_cu);
 }
 
 int smx_admm_set_delta_threshold(double tau) {
@@ -2030,7 +2051,7 @@ int smx_admm_get_delta_threshold(double* tau) {
 
 int smx_admm_get_occupancy(int32_t* workgroups_per_cu) {
   SMX_REQUIRE(workgroups_per_cu, "smx_admm_get_occupancy: null pointer");
-  *workgroups_per_cu = g_admm_occ.load(std::memory_order_relaxed);
+  *workgroups_per_cu = static_cast<int32_t>(knob(KN_ADMM_OCC));
   return 0;
 }
 

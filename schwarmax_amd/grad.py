@@ -326,8 +326,8 @@ def model_likelihood_and_grad_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_
     The models run one after another through model_likelihood_and_grad, each
     freeing its slabs, checkpoints and NNLS workspace before the next (a
     40,000-orbit model holds about 110 GB at its peak).
-    autotune: pick the ADMM product's occupancy for the batch's median shape
-    first (``tune.ensure_tuned``, once per process and shape bucket).
+    autotune: pick the NNLS stages' launch-shape knobs for the batch's median
+    shape first (``tune.ensure_tuned``, once per process and shape bucket).
     """
     thetas = np.asarray(thetas, dtype=np.float64).reshape(-1, 12)
     M = thetas.shape[0]

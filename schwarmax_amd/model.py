@@ -284,16 +284,16 @@ def model_from_library(lib, P, B, params_disk_rho, dict_data, nnls_maxiter, regu
 
 
 def models_from_libraries(libs, P, B, params_disk_rhos, dict_data, nnls_maxiter, regularization,
-                          channel_w):
+                          channel_w, arena=None):
     """model_from_library for several models (one galaxy, the same P and B) with
     their NNLS solves grouped: one ADMM product launch and one update launch
-    per iteration for all of them (``nnls.solve_group``).  Each model's result
-    equals its model_from_library call (nnls_tol = 0)."""
+    per iteration for all of them (``nnls.solve_group``; ``arena`` as there).
+    Each model's result equals its model_from_library call (nnls_tol = 0)."""
     from .nnls import solve_group
     ts = [_bootstrap_data(lib.shape[0], pb, dict_data) for lib, pb in zip(libs, params_disk_rhos)]
     Ws = solve_group(libs, P, B, [(t['y_R'], t['gamma_kin'], t['y_xy_boot'], t['y_h_boot'],
                                    t['sig_R'], t['sig_xy'], t['sig_h']) for t in ts],
-                     regularization, nnls_maxiter, *channel_w)
+                     regularization, nnls_maxiter, *channel_w, arena=arena)
     return [(_model_maps(W, lib, P, B, t), dict(mmpo=t['mmpo'], n=lib.shape[0],
                                                 nnls_iters=int(nnls_maxiter)))
             for W, lib, t in zip(Ws, libs, ts)]

@@ -83,20 +83,35 @@ def solve_nnls_admm_bootstrap_lib(lib, P, B, y_R, gamma_kin, y_xy_boot, y_h_boot
     return W, {'iters': it}
 
 
-def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy=1.0, w_h=1.0):
+def group_bytes(ns, m, K):
+    """Device bytes of a grouped solve of models with orbit counts ns: the
+    models' states and the shared scratch, each rounded up to 256 bytes."""
+    L = _lib.lib()
+    r = lambda b: (int(b) + 255) // 256 * 256  # noqa: E731
+    return (sum(r(L.smx_nnls_group_state_bytes(int(n), m, K)) for n in ns) +
+            r(L.smx_nnls_group_scratch_bytes(max(int(n) for n in ns), m, K)))
+
+
+def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy=1.0, w_h=1.0,
+                arena=None):
     """Bootstrap solves (nnls.py:68-166) of several models in grouped launches
     (``smx_nnls_admm_group``: per ADMM iteration, one product launch and one
     update launch for all models).  libs[g] (n_g, >= P+5B) are the models'
     library columns; data[g] = (y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R,
     sig_xy, sig_h) as for solve_nnls_admm_bootstrap_lib, with one K for all.
+    ``arena``: optional uint8 CUDA tensor of at least ``group_bytes`` bytes
+    from which the states and the scratch are taken (a caller solving many
+    groups keeps one arena instead of allocating per group).
     Returns each model's weights (K, n_g), equal to its single solve (tol 0)."""
     L = _lib.lib()
     if len(libs) != len(data):
         raise ValueError("solve_group: one data tuple per library")
     if len(libs) > _lib.NNLS_GROUP_MAX:
         h = _lib.NNLS_GROUP_MAX
-        return (solve_group(libs[:h], P, B, data[:h], lambda_reg, maxiter, w_rzphi, w_xy, w_h) +
-                solve_group(libs[h:], P, B, data[h:], lambda_reg, maxiter, w_rzphi, w_xy, w_h))
+        return (solve_group(libs[:h], P, B, data[:h], lambda_reg, maxiter, w_rzphi, w_xy, w_h,
+                            arena) +
+                solve_group(libs[h:], P, B, data[h:], lambda_reg, maxiter, w_rzphi, w_xy, w_h,
+                            arena))
     G = len(libs)
     if G == 0:
         return []
@@ -104,6 +119,22 @@ def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy
     f64 = torch.float64
     probs = (_lib.SmxNnlsProblem * G)()
     keep, Ws, K = [], [], None
+    Ks = {int(d[2].shape[0]) for d in data}
+    if len(Ks) != 1:
+        raise ValueError("solve_group: every model needs the same number of right-hand sides")
+    K0 = Ks.pop()
+    need = group_bytes([lib.shape[0] for lib in libs], m, K0)
+    if arena is None:
+        arena = torch.empty(need, dtype=torch.uint8, device='cuda')
+    elif arena.dtype != torch.uint8 or not arena.is_cuda or arena.numel() < need:
+        raise ValueError(f"solve_group: the arena must be a uint8 CUDA tensor of >= {need} bytes")
+    off = 0
+
+    def take(nbytes):
+        nonlocal off
+        t = arena[off:off + int(nbytes)]
+        off += (int(nbytes) + 255) // 256 * 256
+        return t
     for g, (lib, d) in enumerate(zip(libs, data)):
         y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R, sig_xy, sig_h = d
         row_scale, Y = _scales_and_rhs(P, B, y_R, gamma_kin, y_xy_boot, y_h_boot, sig_R, sig_xy,
@@ -116,8 +147,7 @@ def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy
             raise ValueError("solve_group: every model needs the same number of right-hand sides")
         if lib.shape[1] < m:
             raise ValueError("solve_group: library rows shorter than P + 5B")
-        state = torch.empty(int(L.smx_nnls_group_state_bytes(n, m, K)), dtype=torch.uint8,
-                            device='cuda')
+        state = take(L.smx_nnls_group_state_bytes(n, m, K))
         W = torch.empty((K, n), dtype=f64, device='cuda')
         probs[g] = _lib.SmxNnlsProblem(_lib.ptr(lib, f64).value, lib.shape[1], n,
                                        _lib.ptr(row_scale, f64).value, _lib.ptr(Y, f64).value,
@@ -126,8 +156,7 @@ def solve_group(libs, P, B, data, lambda_reg=1.0, maxiter=200, w_rzphi=1.0, w_xy
         keep += [row_scale, Y, state]
         Ws.append(W)
     n_max = max(lib.shape[0] for lib in libs)
-    scratch = torch.empty(int(L.smx_nnls_group_scratch_bytes(n_max, m, K)), dtype=torch.uint8,
-                          device='cuda')
+    scratch = take(L.smx_nnls_group_scratch_bytes(n_max, m, K))
     _lib.check(L.smx_nnls_admm_group(G, probs, m, K, float(lambda_reg), int(maxiter),
                                      _lib.ptr(scratch), _lib.stream_ptr()), 'smx_nnls_admm_group')
     if _lib.TIMING['on']:

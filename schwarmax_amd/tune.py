@@ -1,16 +1,29 @@
-"""Autotuner for the ADMM product's occupancy.
+"""Autotuner for the launch-shape knobs of the NNLS stages.
 
-The ADMM product (one launch per iteration, the largest share of a model
-evaluation) splits its reduction so that the grid holds a chosen number of
-product workgroups per CU (and at least one workgroup per CU).  ``autotune_admm`` times the product kernel at each
-candidate setting on a synthetic problem of the workload's shape, full and
-delta launches weighted by the share of full products a solve takes, and
-keeps the fastest setting in the library (``smx_admm_set_occupancy``).  The
-setting changes only the order of the split-K partial sums (results move by
-FP32 rounding), never the iterate definition.
+The knobs (``smx_tune_set``; process-wide, read by each launch) choose how
+work is laid out, never what is computed:
 
-``ensure_tuned`` tunes once per process and shape bucket; the batched path
-(``batch.model_likelihood_batch(autotune=True)``) calls it.
+* ADMM product occupancy: the workgroups per CU that the product's split-K
+  rule targets (the product is the largest share of a model evaluation);
+* Gram waves: 4 waves of 128 x 64 outputs or 8 waves of 128 x 32 per
+  workgroup;
+* inverse syrkx threshold: the node size from which the FP64 inverse's two
+  symmetric updates use rocBLAS syrkx rather than full dgemm products;
+* FP64 GEMM split target: the workgroups the split-K rule of the FP64 GEMM
+  (right-hand sides c = -Y U and the ADMM anchors) fills.
+
+``autotune_admm`` times the product kernel at each occupancy on a synthetic
+problem of the workload's shape, full and delta launches weighted by the
+share of full products a solve takes.  ``autotune_factor`` times a one-
+iteration solve of the same shape (Gram, inverse, c = -Y U and one FP64
+anchor) under each candidate of the other three knobs, one knob at a time,
+with the library's stage timing.  Both keep the fastest settings.  The
+settings change only layouts and the order of partial sums (results move by
+rounding), never the iterate definition.
+
+``ensure_tuned`` tunes once per process and shape bucket; the batched paths
+(``batch.model_likelihood_batch`` and ``grad.model_likelihood_and_grad_batch``
+with autotune=True) call it.
 """
 import ctypes
 
@@ -24,7 +37,26 @@ from . import _lib
 FULL_SHARE = 0.68
 CANDIDATES = (1, 2, 3, 4, 5, 6, 7, 8)
 
+KNOB_ADMM_OCCUPANCY = 0
+KNOB_GRAM_WAVES = 1
+KNOB_INV_SYRKX_MIN = 2
+KNOB_GEMM64_SPLIT_TARGET = 3
+GRAM_WAVES = (4, 8)
+SYRKX_MIN = (4096, 8192, 16384, 32768)
+SPLIT_TARGETS = (256, 512, 1024, 2048, 4096)
+
 _RESULTS = {}
+
+
+def get_knob(knob):
+    v = ctypes.c_int64(0)
+    _lib.check(_lib.lib().smx_tune_get(int(knob), ctypes.byref(v)), 'smx_tune_get')
+    return int(v.value)
+
+
+def set_knob(knob, value):
+    """0 restores the knob's default."""
+    _lib.check(_lib.lib().smx_tune_set(int(knob), int(value)), 'smx_tune_set')
 
 
 def get_occupancy():
@@ -93,6 +125,67 @@ def autotune_admm(n=20000, m=17288, K=100, candidates=CANDIDATES, reps=10, full_
     return {'occupancy': best, 'ms': results, 'default': default, 'shape': (n, m, K)}
 
 
+def _solve_stages(n, m, K, lib, scale, Y, ws, W):
+    """{stage: (ms, launches)} of a one-iteration solve on the given buffers."""
+    L = _lib.lib()
+    iters = ctypes.c_int32(0)
+    _lib.timing_enable(True)
+    _lib.check(L.smx_nnls_admm(_lib.ptr(lib), m, n, m, _lib.ptr(scale), _lib.ptr(Y), K, 1.0, 1, 1.0,
+                               0.0, _lib.ptr(ws), _lib.ptr(W), ctypes.byref(iters),
+                               _lib.stream_ptr()), 'smx_nnls_admm')
+    torch.cuda.synchronize()
+    return _lib.timing_read()
+
+
+def autotune_factor(n=20000, m=17288, K=100, seed=0, apply=True, repeats=2):
+    """Time the Gram, inverse and FP64-GEMM knobs on a one-iteration solve of
+    shape (n, m, K); keep the fastest value of each (one knob at a time, the
+    others at their current values).  Returns {'gram_waves': best,
+    'syrkx_min': best, 'split_target': best, 'ms': {knob name: {value: ms}}}."""
+    L = _lib.lib()
+    g = torch.Generator(device='cuda')
+    g.manual_seed(seed)
+    lib = torch.rand((n, m), dtype=torch.float64, device='cuda', generator=g)
+    lib *= (torch.rand((n, m), device='cuda', generator=g) < 0.05)
+    scale = torch.ones(m, dtype=torch.float64, device='cuda')
+    Y = torch.rand((K, m), dtype=torch.float64, device='cuda', generator=g)
+    ws = torch.empty(int(L.smx_nnls_workspace_bytes(n, m, K)), dtype=torch.uint8, device='cuda')
+    W = torch.empty((K, n), dtype=torch.float64, device='cuda')
+    was_on = _lib.TIMING['on']
+    knobs = {'gram_waves': (KNOB_GRAM_WAVES, GRAM_WAVES, ('gram',)),
+             # a threshold above n leaves every node on dgemm; below the
+             # smallest node size every node uses syrkx
+             'syrkx_min': (KNOB_INV_SYRKX_MIN,
+                           tuple(sorted({v for v in SYRKX_MIN if v <= n} | {max(n + 1, 256)})),
+                           ('factor',)),
+             'split_target': (KNOB_GEMM64_SPLIT_TARGET, SPLIT_TARGETS, ('rhs', 'admm'))}
+    results, best = {}, {}
+    try:
+        _solve_stages(n, m, K, lib, scale, Y, ws, W)          # warm-up
+        for name, (knob, values, stages) in knobs.items():
+            before = get_knob(knob)
+            res = {}
+            try:
+                for v in values:
+                    set_knob(knob, v)
+                    t = []
+                    for _ in range(repeats):
+                        st = _solve_stages(n, m, K, lib, scale, Y, ws, W)
+                        t.append(sum(st[s][0] for s in stages))
+                    res[int(v)] = min(t)
+            finally:
+                set_knob(knob, before)
+            results[name] = res
+            best[name] = min(res, key=res.get)
+            if apply:
+                set_knob(knob, best[name])
+    finally:
+        _lib.timing_enable(was_on)
+    del lib, ws, W
+    torch.cuda.empty_cache()
+    return dict(best, ms=results, shape=(n, m, K))
+
+
 def _bucket(n, m, K):
     # one tuning per power-of-two bucket of n and of K, per m
     return (int(n).bit_length(), int(m), int(K).bit_length())
@@ -100,10 +193,17 @@ def _bucket(n, m, K):
 
 def ensure_tuned(n, m, K, **kw):
     """Tune once per process and shape bucket; returns that tuning's result
-    (and applies its setting)."""
+    (and applies its settings)."""
     key = _bucket(n, m, K)
     if key not in _RESULTS:
-        _RESULTS[key] = autotune_admm(n, m, K, **kw)
+        res = autotune_admm(n, m, K, **kw)
+        res['factor'] = autotune_factor(n, m, K)
+        _RESULTS[key] = res
     else:
-        set_occupancy(_RESULTS[key]['occupancy'])
+        res = _RESULTS[key]
+        set_occupancy(res['occupancy'])
+        f = res['factor']
+        set_knob(KNOB_GRAM_WAVES, f['gram_waves'])
+        set_knob(KNOB_INV_SYRKX_MIN, f['syrkx_min'])
+        set_knob(KNOB_GEMM64_SPLIT_TARGET, f['split_target'])
     return _RESULTS[key]

@@ -132,3 +132,32 @@ def test_chunked_schedule_rounds_to_whole_chunks():
     assert M.make_schedule(s, 1.0).N_max == 200
     assert M.make_schedule(s, 1.0, chunked=False).N_max == 250
     assert M.make_schedule(M.schedule_of(None), 1.0).N_max == 5000
+
+
+def test_tuning_knobs_without_gpu():
+    """smx_tune_set validates each knob's range and 0 restores the default."""
+    from schwarmax_amd import _lib, tune
+    L = _lib.load_library()
+    v = ctypes.c_int64(0)
+    defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 4,
+                tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048}
+    bad = {tune.KNOB_ADMM_OCCUPANCY: 9, tune.KNOB_GRAM_WAVES: 6, tune.KNOB_INV_SYRKX_MIN: 100,
+           tune.KNOB_GEMM64_SPLIT_TARGET: 8192}
+    good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 8, tune.KNOB_INV_SYRKX_MIN: 4096,
+            tune.KNOB_GEMM64_SPLIT_TARGET: 512}
+    for k in defaults:
+        assert L.smx_tune_get(k, ctypes.byref(v)) == 0
+        before = v.value
+        assert L.smx_tune_set(k, bad[k]) == 2
+        assert L.smx_tune_set(k, good[k]) == 0
+        assert L.smx_tune_get(k, ctypes.byref(v)) == 0 and v.value == good[k]
+        assert L.smx_tune_set(k, 0) == 0
+        assert L.smx_tune_get(k, ctypes.byref(v)) == 0 and v.value == defaults[k]
+        assert L.smx_tune_set(k, before) == 0
+    assert L.smx_tune_set(4, 1) == 2 and L.smx_tune_get(-1, ctypes.byref(v)) == 2
+    # the occupancy entry point is the same knob
+    o = ctypes.c_int32(0)
+    assert L.smx_admm_set_occupancy(5) == 0 and L.smx_admm_get_occupancy(ctypes.byref(o)) == 0
+    assert o.value == 5 and L.smx_tune_get(tune.KNOB_ADMM_OCCUPANCY, ctypes.byref(v)) == 0
+    assert v.value == 5
+    assert L.smx_admm_set_occupancy(0) == 0

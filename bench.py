@@ -221,7 +221,9 @@ def bench_batched(args):
 
     def step():
         return batch.model_likelihood_batch(thetas, dd, B, P, n_orbits=n_orbits, schedule=sched,
-                                            nnls_maxiter=T, nnls_tol=args.nnls_tol)
+                                            nnls_maxiter=T, nnls_tol=args.nnls_tol,
+                                            grouped_nnls=not args.per_model_nnls,
+                                            nnls_budget_bytes=int(args.nnls_budget_gib * 2**30))
     # the launch-shape knobs for the batch's median shape, once per process
     # (the batch call would otherwise tune inside the first timed step)
     from schwarmax_amd import tune
@@ -255,6 +257,8 @@ def bench_batched(args):
                    'admm_iters': T},
         'stages_ms': {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
     }
+    out['nnls_grouping'] = ('model by model' if args.per_model_nnls else
+                            f'grouped solves of <= {args.nnls_budget_gib:g} GiB of state')
     from schwarmax_amd import tune
     out['autotune'] = [{'shape': list(r['shape']), 'occupancy': r['occupancy'],
                         'default': r['default'],
@@ -346,6 +350,10 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--nnls-tol', type=float, default=0.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--nnls-budget-gib', type=float, default=16.0,
+                    help='batched config: solve state per grouped NNLS solve')
+    ap.add_argument('--per-model-nnls', action='store_true',
+                    help='batched config: solve the NNLS model by model')
     ap.add_argument('--no-autotune', action='store_true',
                     help='plain / grad configs: keep the default launch-shape knobs')
     ap.add_argument('--compare', action='store_true',

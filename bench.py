@@ -264,7 +264,7 @@ def bench_batched(args):
                         'default': r['default'],
                         'ms_full_delta_weighted': {str(o): [round(x, 4) for x in v]
                                                    for o, v in r['ms'].items()},
-                        'gram_waves': r['factor']['gram_waves'],
+                        'admm_loads': r['loads'], 'gram_waves': r['factor']['gram_waves'],
                         'inv_syrkx_min': r['factor']['syrkx_min'],
                         'gemm64_split_target': r['factor']['split_target']}
                        for r in tune._RESULTS.values()]
@@ -424,7 +424,8 @@ def main():
         t0 = time.time()
         res = tune.ensure_tuned(n, P + 5 * B, K)
         f = res['factor']
-        tuning = {'admm_occupancy': res['occupancy'], 'gram_waves': f['gram_waves'],
+        tuning = {'admm_occupancy': res['occupancy'], 'admm_loads': res['loads'],
+                  'gram_waves': f['gram_waves'],
                   'inv_syrkx_min': f['syrkx_min'], 'gemm64_split_target': f['split_target']}
         log(f'autotuned {tuning} in {time.time() - t0:.1f}s')
     for i in range(args.warmup):

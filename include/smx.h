@@ -345,13 +345,20 @@ int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
  *                                 smaller ones with full dgemm products.
  *   SMX_KNOB_GEMM64_SPLIT_TARGET  256..4096, default 2048: workgroups the
  *                                 FP64 GEMM's split-K rule fills (right-hand
- *                                 sides c = -Y U, the ADMM anchors). */
+ *                                 sides c = -Y U, the ADMM anchors).
+ *   SMX_KNOB_ADMM_LOADS           0, 1 or 2, default 0: how the ADMM product
+ *                                 loads the inverse's planes: 0 non-temporal
+ *                                 with one 128-row block of right-hand sides
+ *                                 and cached with more (their workgroups share
+ *                                 M's tiles through L2), 1 always
+ *                                 non-temporal, 2 always cached. */
 enum {
   SMX_KNOB_ADMM_OCCUPANCY = 0,
   SMX_KNOB_GRAM_WAVES = 1,
   SMX_KNOB_INV_SYRKX_MIN = 2,
   SMX_KNOB_GEMM64_SPLIT_TARGET = 3,
-  SMX_KNOB_COUNT = 4
+  SMX_KNOB_ADMM_LOADS = 4,
+  SMX_KNOB_COUNT = 5
 };
 int smx_tune_set(int32_t knob, int64_t value);
 int smx_tune_get(int32_t knob, int64_t* value);

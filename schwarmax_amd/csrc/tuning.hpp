@@ -12,6 +12,7 @@ enum Knob {
   KN_GRAM_WAVES,         // waves per Gram workgroup (4 or 8)
   KN_INV_SYRKX_MIN,      // inverse nodes of at least this many rows use rocBLAS syrkx updates
   KN_GEMM64_SPLIT,       // workgroups the FP64 GEMM's split-K rule fills (256..4096)
+  KN_ADMM_LOADS,         // ADMM product's loads of M: 0 auto, 1 non-temporal, 2 cached
   KN_COUNT
 };
 

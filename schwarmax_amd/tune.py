@@ -10,7 +10,9 @@ work is laid out, never what is computed:
 * inverse syrkx threshold: the node size from which the FP64 inverse's two
   symmetric updates use rocBLAS syrkx rather than full dgemm products;
 * FP64 GEMM split target: the workgroups the split-K rule of the FP64 GEMM
-  (right-hand sides c = -Y U and the ADMM anchors) fills.
+  (right-hand sides c = -Y U and the ADMM anchors) fills;
+* ADMM loads: how the product loads the inverse's planes (non-temporal,
+  cached, or by the automatic rule on the planes' size and K).
 
 ``autotune_admm`` times the product kernel at each occupancy on a synthetic
 problem of the workload's shape, full and delta launches weighted by the
@@ -41,6 +43,7 @@ KNOB_ADMM_OCCUPANCY = 0
 KNOB_GRAM_WAVES = 1
 KNOB_INV_SYRKX_MIN = 2
 KNOB_GEMM64_SPLIT_TARGET = 3
+KNOB_ADMM_LOADS = 4
 GRAM_WAVES = (4, 8)
 SYRKX_MIN = (4096, 8192, 16384, 32768)
 SPLIT_TARGETS = (256, 512, 1024, 2048, 4096)
@@ -100,29 +103,49 @@ def time_product(n, m, K, ws, full, reps):
 
 def autotune_admm(n=20000, m=17288, K=100, candidates=CANDIDATES, reps=10, full_share=FULL_SHARE,
                   seed=0, apply=True):
-    """Time the ADMM product at each occupancy candidate; keep the fastest.
+    """Time the ADMM product at each occupancy candidate (loads of M by the
+    automatic policy); keep the fastest.  Then time the two explicit load
+    policies at that occupancy and keep one only if it beats the automatic
+    policy by more than 3% (the automatic one also adapts to other shapes).
 
-    Returns {'occupancy': best, 'ms': {occ: (full, delta, weighted)}, 'default': ...}.
+    Returns {'occupancy': best, 'ms': {occ: (full, delta, weighted)},
+    'loads': 0 | 1 | 2, 'loads_ms': {policy: weighted}, 'default': ...}.
     """
     default = 2
     before = get_occupancy()
+    loads_before = get_knob(KNOB_ADMM_LOADS)
     ws = _synthetic_workspace(n, m, K, seed)
-    results = {}
+    results, lres = {}, {}
+
+    def weighted():
+        time_product(n, m, K, ws, True, 1)                  # first launch at this setting
+        t_full = time_product(n, m, K, ws, True, reps)
+        t_delta = time_product(n, m, K, ws, False, reps)
+        return (t_full, t_delta, full_share * t_full + (1 - full_share) * t_delta)
+
     try:
+        set_knob(KNOB_ADMM_LOADS, 0)
         for occ in candidates:
             set_occupancy(occ)
-            time_product(n, m, K, ws, True, 1)              # first launch at this grid
-            t_full = time_product(n, m, K, ws, True, reps)
-            t_delta = time_product(n, m, K, ws, False, reps)
-            results[int(occ)] = (t_full, t_delta, full_share * t_full + (1 - full_share) * t_delta)
+            results[int(occ)] = weighted()
+        best = min(results, key=lambda o: results[o][2])
+        set_occupancy(best)
+        lres[0] = results[best][2]
+        for v in (1, 2):
+            set_knob(KNOB_ADMM_LOADS, v)
+            lres[v] = weighted()[2]
     finally:
         set_occupancy(before)
+        set_knob(KNOB_ADMM_LOADS, loads_before)
     del ws
     torch.cuda.empty_cache()
-    best = min(results, key=lambda o: results[o][2])
+    loads = min((1, 2), key=lres.get)
+    loads = loads if lres[loads] < 0.97 * lres[0] else 0
     if apply:
         set_occupancy(best)
-    return {'occupancy': best, 'ms': results, 'default': default, 'shape': (n, m, K)}
+        set_knob(KNOB_ADMM_LOADS, loads)
+    return {'occupancy': best, 'ms': results, 'loads': loads, 'loads_ms': lres,
+            'default': default, 'shape': (n, m, K)}
 
 
 def _solve_stages(n, m, K, lib, scale, Y, ws, W):
@@ -202,6 +225,7 @@ def ensure_tuned(n, m, K, **kw):
     else:
         res = _RESULTS[key]
         set_occupancy(res['occupancy'])
+        set_knob(KNOB_ADMM_LOADS, res['loads'])
         f = res['factor']
         set_knob(KNOB_GRAM_WAVES, f['gram_waves'])
         set_knob(KNOB_INV_SYRKX_MIN, f['syrkx_min'])

@@ -140,11 +140,12 @@ def test_tuning_knobs_without_gpu():
     L = _lib.load_library()
     v = ctypes.c_int64(0)
     defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 4,
-                tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048}
+                tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048,
+                tune.KNOB_ADMM_LOADS: 0}
     bad = {tune.KNOB_ADMM_OCCUPANCY: 9, tune.KNOB_GRAM_WAVES: 6, tune.KNOB_INV_SYRKX_MIN: 100,
-           tune.KNOB_GEMM64_SPLIT_TARGET: 8192}
+           tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3}
     good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 8, tune.KNOB_INV_SYRKX_MIN: 4096,
-            tune.KNOB_GEMM64_SPLIT_TARGET: 512}
+            tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2}
     for k in defaults:
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0
         before = v.value
@@ -154,7 +155,7 @@ def test_tuning_knobs_without_gpu():
         assert L.smx_tune_set(k, 0) == 0
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0 and v.value == defaults[k]
         assert L.smx_tune_set(k, before) == 0
-    assert L.smx_tune_set(4, 1) == 2 and L.smx_tune_get(-1, ctypes.byref(v)) == 2
+    assert L.smx_tune_set(5, 1) == 2 and L.smx_tune_get(-1, ctypes.byref(v)) == 2
     # the occupancy entry point is the same knob
     o = ctypes.c_int32(0)
     assert L.smx_admm_set_occupancy(5) == 0 and L.smx_admm_get_occupancy(ctypes.byref(o)) == 0

@@ -22,14 +22,17 @@ def test_autotune_and_solve():
         for occ, (tf, td, tw) in res['ms'].items():
             assert tf > 0 and td > 0 and min(tf, td) <= tw <= max(tf, td)
         assert tune.get_occupancy() == res['occupancy']
+        assert res['loads'] in (0, 1, 2) and set(res['loads_ms']) == {0, 1, 2}
+        assert tune.get_knob(tune.KNOB_ADMM_LOADS) == res['loads']
         # every setting solves the same problem to the oracle's bars
         A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy, sh = _problem(2, n=300, K=12)
         w5 = np.sqrt(5.0)
         ref = ON.solve_nnls_admm_bootstrap(A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy,
                                            sh, lambda_reg=1.0, maxiter=300, w_rzphi=w5, w_xy=w5,
                                            w_h=1.0)
-        for occ in tune.CANDIDATES:
+        for occ, loads in [(o, 0) for o in tune.CANDIDATES] + [(2, 1), (2, 2)]:
             tune.set_occupancy(occ)
+            tune.set_knob(tune.KNOB_ADMM_LOADS, loads)
             got = GN.solve_nnls_admm_bootstrap(A_R, A_xy, *A_h, y_R, y_xy, gamma, yb_xy, *yb_h, sR,
                                                sxy, *sh, lambda_reg=1.0, maxiter=300, w_rzphi=w5,
                                                w_xy=w5, w_h=1.0).cpu().numpy()
@@ -39,6 +42,7 @@ def test_autotune_and_solve():
             tune.set_occupancy(9)
     finally:
         tune.set_occupancy(before)
+        tune.set_knob(tune.KNOB_ADMM_LOADS, 0)
 
 
 def test_autotune_factor_knobs():

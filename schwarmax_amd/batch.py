@@ -46,8 +46,8 @@ def _groups(n_orbits, row_bytes, budget):
 
 def _nnls_groups(models, n_orbits, m_rows, K, budget):
     """Consecutive runs of models whose NNLS solve states fit the budget
-    (at most NNLS_GROUP_MAX per run, at least one)."""
-    L = _lib.lib()
+    (at most NNLS_GROUP_MAX per run, at least one; a size query, no GPU)."""
+    L = _lib.load_library()
     groups, cur, size = [], [], 0
     for mi in models:
         b = int(L.smx_nnls_group_state_bytes(int(n_orbits[mi]), int(m_rows), int(K)))

@@ -85,8 +85,9 @@ def solve_nnls_admm_bootstrap_lib(lib, P, B, y_R, gamma_kin, y_xy_boot, y_h_boot
 
 def group_bytes(ns, m, K):
     """Device bytes of a grouped solve of models with orbit counts ns: the
-    models' states and the shared scratch, each rounded up to 256 bytes."""
-    L = _lib.lib()
+    models' states and the shared scratch, each rounded up to 256 bytes (a
+    size query, no GPU)."""
+    L = _lib.load_library()
     r = lambda b: (int(b) + 255) // 256 * 256  # noqa: E731
     return (sum(r(L.smx_nnls_group_state_bytes(int(n), m, K)) for n in ns) +
             r(L.smx_nnls_group_scratch_bytes(max(int(n) for n in ns), m, K)))

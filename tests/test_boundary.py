@@ -162,3 +162,31 @@ def test_tuning_knobs_without_gpu():
     assert o.value == 5 and L.smx_tune_get(tune.KNOB_ADMM_OCCUPANCY, ctypes.byref(v)) == 0
     assert v.value == 5
     assert L.smx_admm_set_occupancy(0) == 0
+
+
+def test_nnls_groups_respect_budget_and_order():
+    """The batched path's NNLS groups: consecutive runs of the models, each
+    within the state budget unless a single model exceeds it, at most
+    NNLS_GROUP_MAX per run (size queries only, no GPU)."""
+    from schwarmax_amd import _lib, batch, nnls
+    L = _lib.load_library()
+    m, K = 17288, 100
+    rng = np.random.default_rng(3)
+    n_orbits = [int(v) for v in rng.integers(5000, 40001, size=70)] + [100] * 80
+    models = list(range(len(n_orbits)))
+    state = {i: L.smx_nnls_group_state_bytes(n_orbits[i], m, K) for i in models}
+    for budget in (1, 16 << 30, 80 << 30, 1 << 50):
+        groups = batch._nnls_groups(models, n_orbits, m, K, budget)
+        assert [i for g in groups for i in g] == models
+        for g in groups:
+            assert 1 <= len(g) <= _lib.NNLS_GROUP_MAX
+            assert len(g) == 1 or sum(state[i] for i in g) <= budget
+        if budget == 1:
+            assert all(len(g) == 1 for g in groups)
+        if budget == 1 << 50:
+            assert [len(g) for g in groups] == [64, 64, 22]
+    # a group's device bytes: its states plus the scratch of its largest model
+    g = [0, 1, 2]
+    need = nnls.group_bytes([n_orbits[i] for i in g], m, K)
+    assert need >= sum(state[i] for i in g) + L.smx_nnls_group_scratch_bytes(
+        max(n_orbits[i] for i in g), m, K)

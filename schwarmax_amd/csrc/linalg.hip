@@ -295,19 +295,24 @@ __global__ void scale_cols_kernel(const double* __restrict__ src, int64_t rows, 
 // The ADMM iterate is kept as one FP64 array x = w_hat + u_old: then
 // z = max(0, x) and u = min(0, x) exactly, as nnls.py:155-156 compute them
 // (z = max(0, x); u = (u_old + w_hat) - z is x - x = 0 or x - 0 = x).
-__global__ void zmax_kernel(int K, int64_t n, const double* __restrict__ x, unsigned* __restrict__ zmax) {
+// x has rows of nx = n rounded up to 4 (the update's aligned vectors).
+__global__ void zmax_kernel(int K, int64_t n, int64_t nx, const double* __restrict__ x,
+                            unsigned* __restrict__ zmax) {
   const int k = blockIdx.y;
   float m = 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    m = fmaxf(m, static_cast<float>(fmax(0.0, x[(int64_t)k * n + i])));
+    m = fmaxf(m, static_cast<float>(fmax(0.0, x[(int64_t)k * nx + i])));
   if (m > 0.f) atomicMax(zmax + k, __float_as_uint(m));
 }
 
-// W = z = max(0, x) after at least one iteration, z_init = w_init before any.
-__global__ void copy_out_kernel(int64_t total, const double* __restrict__ x, int iterated,
-                                double w_init, double* __restrict__ W) {
+// W (K, n) = z = max(0, x) after at least one iteration, z_init = w_init
+// before any; x has rows of nx.
+__global__ void copy_out_kernel(int64_t K, int64_t n, int64_t nx, const double* __restrict__ x,
+                                int iterated, double w_init, double* __restrict__ W) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < total) W[i] = iterated ? fmax(0.0, x[i]) : w_init;
+  if (i >= K * n) return;
+  const int64_t k = i / n;
+  W[i] = iterated ? fmax(0.0, x[k * nx + (i - k * n)]) : w_init;
 }
 
 // ---------------------------------------------------------------------------
@@ -672,6 +677,9 @@ __global__ __launch_bounds__(AG_NT, AG_OCC) void admm_gemm_kernel(
 
 __device__ __forceinline__ float block_max_256(float v, float* red);
 
+// Row stride of the ADMM's K x n FP64 arrays: n rounded up to 4.
+__host__ __device__ __forceinline__ int64_t align_up4(int64_t n) { return (n + 3) / 4 * 4; }
+
 // Index of R[k][i] in the blocked FP32 layout the product reads.
 __device__ __forceinline__ int64_t r_tile_index(int64_t k, int64_t i, int64_t Kp) {
   return ((i / AG_KP) * Kp + k) * AG_KP + i % AG_KP;
@@ -829,9 +837,12 @@ __host__ __device__ inline AdmmIter admm_iter(float* R, float* Rb, float* D, dou
 // refreshed, so the operand R - R_a is zero): the partial sums count as 0 and
 // the iteration counts as a full product (new base R_b = 0, w_b = 0).
 // One block of the update: elements [256 V bx, 256 V (bx + 1)) of row k.
+// The K x n FP64 arrays (x, c, W_c, R_a, w_b) have rows of nx = n rounded up
+// to 4, so every 4-element vector is aligned; the elements past n are
+// padding: masked to zero here (x, R, D) and left out of every maximum.
 template <int V>
 __device__ __forceinline__ void admm_update_body(
-    int64_t bx, int64_t k, int64_t K, int64_t n, int64_t Kp, int64_t npad, int S,
+    int64_t bx, int64_t k, int64_t K, int64_t n, int64_t nx, int64_t Kp, int64_t npad, int S,
     const float* __restrict__ part, const double* __restrict__ rho_p, const double* __restrict__ wc,
     double* __restrict__ x, int first, int force, int anchor, float tau, double w_init,
     const AdmmIter& b, double alpha, const int* __restrict__ active, unsigned* __restrict__ delta) {
@@ -864,14 +875,14 @@ __device__ __forceinline__ void admm_update_body(
     } else {
       load_vec<V>(b.Rb + ri, rb);
     }
-    const int64_t idx = k * n + i0;
+    const int64_t idx = k * nx + i0;
     if (!active[k]) {
       double xo[V];
       load_vec<V>(x + idx, xo);
 #pragma unroll
       for (int e = 0; e < V; ++e) {
         rn[e] = rcur[e];
-        fmx = fmaxf(fmx, static_cast<float>(*rho_p * fabs(xo[e])));
+        if (i0 + e < n) fmx = fmaxf(fmx, static_cast<float>(*rho_p * fabs(xo[e])));
       }
     } else {
       const float sM = pow2_scale_for(static_cast<float>(1.0 / *rho_p), 13);
@@ -910,9 +921,14 @@ __device__ __forceinline__ void admm_update_body(
         const double uo = first ? 0.0 : fmin(0.0, xo[e]);
         float dz;
         const double rf = st.apply(w[e], zo, uo, xn[e], dz);
-        rn[e] = static_cast<float>(rf - b.Ra[idx + e]);
-        fmx = fmaxf(fmx, static_cast<float>(fabs(rf)));
-        zmv = fmaxf(zmv, dz);
+        if (i0 + e < n) {
+          rn[e] = static_cast<float>(rf - b.Ra[idx + e]);
+          fmx = fmaxf(fmx, static_cast<float>(fabs(rf)));
+          zmv = fmaxf(zmv, dz);
+        } else {
+          xn[e] = 0.0;
+          rn[e] = 0.f;
+        }
       }
       store_vec<V>(x + idx, xn);
       store_vec<V>(b.R + ri, rn);
@@ -946,8 +962,8 @@ __global__ __launch_bounds__(256) void admm_update_kernel(
     const double* __restrict__ rho_p, const double* __restrict__ wc, double* __restrict__ x,
     int first, int force, int anchor, float tau, double w_init, AdmmIter b, double alpha,
     const int* __restrict__ active, unsigned* __restrict__ delta) {
-  admm_update_body<V>(blockIdx.x, blockIdx.y, K, n, Kp, npad, S, part, rho_p, wc, x, first, force,
-                      anchor, tau, w_init, b, alpha, active, delta);
+  admm_update_body<V>(blockIdx.x, blockIdx.y, K, n, align_up4(n), Kp, npad, S, part, rho_p, wc, x,
+                      first, force, anchor, tau, w_init, b, alpha, active, delta);
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,8 +1025,8 @@ __global__ __launch_bounds__(256) void admm_update_group_kernel(
   const AdmmModel& d = mods[admm_model_of(mods, G, blockIdx.x, false)];
   const int64_t bx = (int64_t)blockIdx.x - d.ublk0;
   const AdmmIter ib = admm_iter(d.Rp, d.Rb, d.D, d.wb, d.Ra, d.rmax, d.full_count, Kp, it);
-  admm_update_body<V>(bx, blockIdx.y, K, d.n, Kp, d.npad, d.S, d.part, d.rho64, d.Wc, d.x, first,
-                      force, anchor, tau, d.w_init, ib, alpha, d.active, d.delta);
+  admm_update_body<V>(bx, blockIdx.y, K, d.n, align_up4(d.n), Kp, d.npad, d.S, d.part, d.rho64, d.Wc,
+                      d.x, first, force, anchor, tau, d.w_init, ib, alpha, d.active, d.delta);
 }
 
 // Anchor refresh (before the product of an iteration): R_a = rho (z - u) =
@@ -1028,7 +1044,7 @@ __global__ __launch_bounds__(256) void admm_anchor_kernel(int64_t K, int64_t n, 
   const int64_t k = blockIdx.y;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t idx = k * n + i;
+  const int64_t idx = k * align_up4(n) + i;
   const double rf = *rho_p * (first ? w_init : fabs(x[idx]));
   Ra[idx] = rf;
   Va[idx] = rf - C[idx];
@@ -1066,6 +1082,7 @@ struct NnlsWs {
   unsigned* rmax;   // 9 x Kp: three rotating row-maximum sets of the operand |R - R_a|,
                     // three of |D|, three of |R| = rho |x|
   int64_t npad, Kp;
+  int64_t nx;     // row stride of C, W_c, R_a, V_a, x, w_b: n rounded up to 4
   int S;
   int64_t scratch_bytes, state_bytes, bytes;
 };
@@ -1160,11 +1177,12 @@ static NnlsWs carve_parts(void* scratch, void* state, int64_t n, int64_t m, int6
   off = 0;
   w.Q64 = reinterpret_cast<double*>(take(n * n * 8));
   w.Ys = reinterpret_cast<double*>(take(K * ldu * 8));
-  w.C = reinterpret_cast<double*>(take(K * n * 8));
-  w.Wc = reinterpret_cast<double*>(take(K * n * 8));
-  w.Ra = reinterpret_cast<double*>(take(K * n * 8));
-  w.Va = reinterpret_cast<double*>(take(K * n * 8));
-  w.x = reinterpret_cast<double*>(take(K * n * 8));
+  w.nx = align_up4(n);
+  w.C = reinterpret_cast<double*>(take(K * w.nx * 8));
+  w.Wc = reinterpret_cast<double*>(take(K * w.nx * 8));
+  w.Ra = reinterpret_cast<double*>(take(K * w.nx * 8));
+  w.Va = reinterpret_cast<double*>(take(K * w.nx * 8));
+  w.x = reinterpret_cast<double*>(take(K * w.nx * 8));
   w.rho = reinterpret_cast<float*>(take(4));
   w.rho64 = reinterpret_cast<double*>(take(8));
   w.delta = reinterpret_cast<unsigned*>(take(K * 4));
@@ -1181,7 +1199,7 @@ static NnlsWs carve_parts(void* scratch, void* state, int64_t n, int64_t m, int6
   w.rmax = reinterpret_cast<unsigned*>(take(9 * w.Kp * 4));
   w.Rb = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
   w.D = reinterpret_cast<float*>(take(w.Kp * w.npad * 4));
-  w.wb = reinterpret_cast<double*>(take(K * n * 8));
+  w.wb = reinterpret_cast<double*>(take(K * w.nx * 8));
   w.full_count = reinterpret_cast<unsigned*>(take(4));
   w.state_bytes = off;
   w.bytes = w.scratch_bytes + w.state_bytes;
@@ -1712,7 +1730,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
   hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, Y, K, m,
                      row_scale, ldu, w.Ys);
   SMX_LAUNCHED();
-  if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, lib, ld_lib, 0.0, w.C, n, 0, w.work64,
+  if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, lib, ld_lib, 0.0, w.C, w.nx, 0, w.work64,
                               n * n + n))
     return e;
   const int64_t tot = K * n;
@@ -1751,7 +1769,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
       hipLaunchKernelGGL(admm_anchor_kernel, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0, st,
                          K, n, Kp, first, w_init, w.rho64, w.x, w.C, w.Ra, w.Va, w.Rp);
       SMX_LAUNCHED();
-      if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n, 0, w.work64,
+      if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, w.nx, w.Q64, n, 0.0, w.Wc, w.nx, 0, w.work64,
                                   n * n + n))
         return e;
     } else {
@@ -1761,19 +1779,15 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
       SMX_LAUNCHED();
       stage_end(ST_ADMM_PRODUCT, st, 1);
     }
-    if (n % 4 == 0)
-      hipLaunchKernelGGL(admm_update_kernel<4>, dim3(blocks_for(n / 4, 256), (unsigned)K), dim3(256),
-                         0, st, K, n, Kp, npad, S, w.part, w.rho64, w.Wc, w.x, first, force, anchor,
-                         tau, w_init, ib, 1.6, w.active, w.delta);
-    else
-      hipLaunchKernelGGL(admm_update_kernel<1>, dim3(blocks_for(n, 256), (unsigned)K), dim3(256), 0,
-                         st, K, n, Kp, npad, S, w.part, w.rho64, w.Wc, w.x, first, force, anchor,
-                         tau, w_init, ib, 1.6, w.active, w.delta);
+    hipLaunchKernelGGL(admm_update_kernel<4>, dim3(blocks_for(w.nx / 4, 256), (unsigned)K), dim3(256),
+                       0, st, K, n, Kp, npad, S, w.part, w.rho64, w.Wc, w.x, first, force, anchor,
+                       tau, w_init, ib, 1.6, w.active, w.delta);
     SMX_LAUNCHED();
     done = it + 1;
     if (chk) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.zmax, K, 0u);
-      hipLaunchKernelGGL(zmax_kernel, dim3(64, (unsigned)K), dim3(256), 0, st, (int)K, n, w.x, w.zmax);
+      hipLaunchKernelGGL(zmax_kernel, dim3(64, (unsigned)K), dim3(256), 0, st, (int)K, n, w.nx, w.x,
+                         w.zmax);
       SMX_HIP(hipMemcpyAsync(dl.data(), w.delta, K * 4, hipMemcpyDeviceToHost, st));
       SMX_HIP(hipMemcpyAsync(zm.data(), w.zmax, K * 4, hipMemcpyDeviceToHost, st));
       SMX_HIP(hipStreamSynchronize(st));
@@ -1790,7 +1804,7 @@ int smx_nnls_admm(const double* lib, int64_t ld_lib, int64_t n, int64_t m, const
     }
   }
   stage_end(ST_ADMM, st, done);
-  hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, w.x,
+  hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, K, n, w.nx, w.x,
                      done > 0 ? 1 : 0, w_init, W);
   SMX_LAUNCHED();
   if (iters_done) *iters_done = done;
@@ -1851,7 +1865,7 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
     hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks_for(K * ldu, 256)), dim3(256), 0, st, p.Y, K,
                        m, p.row_scale, ldu, w.Ys);
     SMX_LAUNCHED();
-    if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, p.lib, p.ld_lib, 0.0, w.C, n, 0,
+    if (int e = gemm_nt<double>(st, K, n, m, -1.0, w.Ys, ldu, p.lib, p.ld_lib, 0.0, w.C, w.nx, 0,
                                 w.work64, n * n + n))
       return e;
     hipLaunchKernelGGL(split_m_kernel, dim3(blocks_for(npad * npad / 8, 256)), dim3(256), 0, st,
@@ -1868,9 +1882,6 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
     stage_end(ST_RHS, st, 4);
   }
   // the models' descriptors and block runs
-  bool v4 = true;
-  for (int g = 0; g < G; ++g) v4 = v4 && probs[g].n % 4 == 0;
-  const int V = v4 ? 4 : 1;
   // Product runs longest workgroups first (chunks per split, descending):
   // the launch's last round then holds the short workgroups of small models
   // rather than a tail of long ones.  The descriptors are stored in run order.
@@ -1891,7 +1902,7 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
     d.pblk0 = pb;
     d.pnblk = (w.npad / AG_BN) * w.S * (Kp / AG_KR);
     d.ublk0 = ub;
-    d.unblk = blocks_for(d.n / V, 256);
+    d.unblk = blocks_for(w.nx / 4, 256);
     pb += align_up(d.pnblk, 8);
     ub += d.unblk;
     d.Rp = w.Rp;
@@ -1933,7 +1944,7 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
                            st, K, n, Kp, first, probs[g].w_init, w.rho64, w.x, w.C, w.Ra, w.Va,
                            w.Rp);
         SMX_LAUNCHED();
-        if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, n, w.Q64, n, 0.0, w.Wc, n, 0,
+        if (int e = gemm_nt<double>(st, K, n, n, 1.0, w.Va, w.nx, w.Q64, n, 0.0, w.Wc, w.nx, 0,
                                     w.work64, n * n + n))
           return e;
       }
@@ -1948,19 +1959,15 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* probs, int64_t
       SMX_LAUNCHED();
       stage_end(ST_ADMM_PRODUCT, st, 1);
     }
-    if (V == 4)
-      hipLaunchKernelGGL(admm_update_group_kernel<4>, dim3((unsigned)ub, (unsigned)K), dim3(256), 0,
-                         st, dmods, G, K, Kp, it, first, force, anchor, tau, 1.6);
-    else
-      hipLaunchKernelGGL(admm_update_group_kernel<1>, dim3((unsigned)ub, (unsigned)K), dim3(256), 0,
-                         st, dmods, G, K, Kp, it, first, force, anchor, tau, 1.6);
+    hipLaunchKernelGGL(admm_update_group_kernel<4>, dim3((unsigned)ub, (unsigned)K), dim3(256), 0, st,
+                       dmods, G, K, Kp, it, first, force, anchor, tau, 1.6);
     SMX_LAUNCHED();
   }
   stage_end(ST_ADMM, st, maxiter);
   for (int g = 0; g < G; ++g) {
     const int64_t tot = K * probs[g].n;
-    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, tot, ws[g].x,
-                       maxiter > 0 ? 1 : 0, probs[g].w_init, probs[g].W);
+    hipLaunchKernelGGL(copy_out_kernel, dim3(blocks_for(tot, 256)), dim3(256), 0, st, K, probs[g].n,
+                       ws[g].nx, ws[g].x, maxiter > 0 ? 1 : 0, probs[g].w_init, probs[g].W);
   }
   SMX_LAUNCHED();
   return 0;

@@ -266,6 +266,7 @@ def bench_batched(args):
                                                    for o, v in r['ms'].items()},
                         'admm_loads': r['loads'], 'gram_waves': r['factor']['gram_waves'],
                         'inv_syrkx_min': r['factor']['syrkx_min'],
+                        'gemm64_kernel': r['factor']['gemm64_kernel'],
                         'gemm64_split_target': r['factor']['split_target']}
                        for r in tune._RESULTS.values()]
     if args.compare:
@@ -426,7 +427,8 @@ def main():
         f = res['factor']
         tuning = {'admm_occupancy': res['occupancy'], 'admm_loads': res['loads'],
                   'gram_waves': f['gram_waves'],
-                  'inv_syrkx_min': f['syrkx_min'], 'gemm64_split_target': f['split_target']}
+                  'inv_syrkx_min': f['syrkx_min'], 'gemm64_kernel': f['gemm64_kernel'],
+                  'gemm64_split_target': f['split_target']}
         log(f'autotuned {tuning} in {time.time() - t0:.1f}s')
     for i in range(args.warmup):
         t0 = time.time()

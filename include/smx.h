@@ -344,21 +344,30 @@ int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
  *                                 their symmetric updates with rocBLAS syrkx,
  *                                 smaller ones with full dgemm products.
  *   SMX_KNOB_GEMM64_SPLIT_TARGET  256..4096, default 2048: workgroups the
- *                                 FP64 GEMM's split-K rule fills (right-hand
- *                                 sides c = -Y U, the ADMM anchors).
+ *                                 tiled FP64 GEMM's split-K rule fills
+ *                                 (right-hand sides c = -Y U and the ADMM
+ *                                 anchors, when they take the tiled kernel).
  *   SMX_KNOB_ADMM_LOADS           0, 1 or 2, default 0: how the ADMM product
  *                                 loads the inverse's planes: 0 non-temporal
  *                                 with one 128-row block of right-hand sides
- *                                 and cached with more (their workgroups share
- *                                 M's tiles through L2), 1 always
- *                                 non-temporal, 2 always cached. */
+ *                                 and a hi plane above 192 MiB, cached
+ *                                 otherwise (row blocks then share M's tiles
+ *                                 through L2, or the plane stays in the
+ *                                 memory-side cache), 1 always non-temporal,
+ *                                 2 always cached.
+ *   SMX_KNOB_GEMM64_KERNEL        1 or 2, default 1: the kernel of the FP64
+ *                                 products with few rows (c = -Y U, the ADMM
+ *                                 anchors): 1 the thin kernel (operands
+ *                                 straight into registers, one wave per SIMD),
+ *                                 2 the 128 x 128-tiled kernel with split-K. */
 enum {
   SMX_KNOB_ADMM_OCCUPANCY = 0,
   SMX_KNOB_GRAM_WAVES = 1,
   SMX_KNOB_INV_SYRKX_MIN = 2,
   SMX_KNOB_GEMM64_SPLIT_TARGET = 3,
   SMX_KNOB_ADMM_LOADS = 4,
-  SMX_KNOB_COUNT = 5
+  SMX_KNOB_GEMM64_KERNEL = 5,
+  SMX_KNOB_COUNT = 6
 };
 int smx_tune_set(int32_t knob, int64_t value);
 int smx_tune_get(int32_t knob, int64_t* value);
@@ -385,6 +394,12 @@ int smx_gemm_nt_f32(int64_t M, int64_t N, int64_t K, float alpha, const float* A
                     const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream);
 int smx_gemm_nt_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream);
+/* C = alpha A B^T in FP64 as the NNLS forms c = -Y U and its anchors: with
+ * work (work_elems doubles) for split-K partials, on the kernel that
+ * SMX_KNOB_GEMM64_KERNEL selects (FP64 partials summed in split order). */
+int smx_gemm_nt_f64_split(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                          double* work, int64_t work_elems, void* stream);
 /* In-place inverse of a full symmetric positive definite n x n FP64 matrix
  * (recursive Schur complements); work holds n*n + n doubles. */
 int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream);

@@ -146,6 +146,7 @@ _SIGS = [
     ('smx_gemm_nt_f32', ctypes.c_int, [_I64, _I64, _I64, ctypes.c_float, _P, _I64, _P, _I64,
                                        ctypes.c_float, _P, _I64, _P]),
     ('smx_gemm_nt_f64', ctypes.c_int, [_I64, _I64, _I64, _D, _P, _I64, _P, _I64, _D, _P, _I64, _P]),
+    ('smx_gemm_nt_f64_split', ctypes.c_int, [_I64, _I64, _I64, _D, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     ('smx_spd_inverse_f64', ctypes.c_int, [_P, _I64, _P, _P]),
     ('smx_model_maps', ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _P, _P]),
 ]

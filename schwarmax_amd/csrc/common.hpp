@@ -51,4 +51,12 @@ __device__ __forceinline__ double nmin(double a, double b) {
   return (a != a) ? a : ((b != b) ? b : (a < b ? a : b));
 }
 
+// Bijection of [0, G): block b, dispatched to XCD b % 8, gets the index that
+// makes each XCD's blocks one contiguous run (the first G % 8 XCDs take
+// G / 8 + 1 blocks, the rest G / 8).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
+  const int64_t q = G / 8, r = G % 8, x = b % 8, slot = b / 8;
+  return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 }  // namespace smx

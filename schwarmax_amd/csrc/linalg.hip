@@ -28,6 +28,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <algorithm>
+#include <type_traits>
 
 #include <mutex>
 #include <atomic>
@@ -36,6 +37,7 @@
 #include "common.hpp"
 #include "timing.hpp"
 #include "tuning.hpp"
+#include "gemm64.hpp"
 #include "../../include/smx.h"
 
 namespace smx {
@@ -59,6 +61,7 @@ namespace smx {
 // ---------------------------------------------------------------------------
 constexpr int NT_TM = 128, NT_TN = 128, NT_TK = 32;
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 typedef float f32x16m __attribute__((ext_vector_type(16)));
 
 template <class T> struct NtMfma;
@@ -238,6 +241,13 @@ static int gemm_nt(hipStream_t st, int64_t M, int64_t N, int64_t K, T alpha, con
                    const T* B, int64_t ldb, T beta, T* C, int64_t ldc, int lower_only = 0,
                    T* work = nullptr, int64_t work_elems = 0) {
   if (M <= 0 || N <= 0) return 0;
+  if constexpr (std::is_same<T, double>::value) {
+    // few rows, a workspace for split partials, 16-byte aligned rows: the thin kernel
+    if (work && !lower_only && beta == 0.0 && K > 0 && M <= GEMM_THIN_MAX_ROWS && lda % 2 == 0 &&
+        ldb % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(B) % 16 == 0 && knob(KN_GEMM64_KERNEL) == 1)
+      return gemm_thin64(st, M, N, K, alpha, A, lda, B, ldb, C, ldc, work, work_elems);
+  }
   constexpr int V = 16 / (int)sizeof(T);
   const bool vec = (lda % V == 0) && (ldb % V == 0) && (K % V == 0) &&
                    (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0);
@@ -594,14 +604,6 @@ __device__ __forceinline__ void admm_gemm_loop(int64_t K, int64_t Kp, int64_t nc
     admm_step<FULL, NT, P>(1, chunk(q + 3), A, Kp, k0, Mh, Ml, jb0, nch, Ah, Al, a, b1, acc);
   }
   if (q < nq) admm_mfma_chunk<FULL, P>(Ah[0], Al[0], b0, acc);   // odd count: the last chunk
-}
-
-// Bijection of [0, G): block b, dispatched to XCD b % 8, gets the index that
-// makes each XCD's blocks one contiguous run (the first G % 8 XCDs take
-// G / 8 + 1 blocks, the rest G / 8).
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
-  const int64_t q = G / 8, r = G % 8, x = b % 8, slot = b / 8;
-  return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
 // One ADMM product: the full one on R (scales from rmax), or the delta one on
@@ -2175,6 +2177,17 @@ int smx_gemm_nt_f64(int64_t M, int64_t N, int64_t K, double alpha, const double*
                   K < (1LL << 31),
               "smx_gemm_nt_f64: bad arguments");
   return gemm_nt<double>(as_stream(stream), M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int smx_gemm_nt_f64_split(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                          double* work, int64_t work_elems, void* stream) {
+  SMX_REQUIRE(A && B && C && work && M >= 0 && N >= 0 && K >= 0 && M < (1LL << 31) &&
+                  N < (1LL << 31) && K < (1LL << 31) && lda >= K && ldb >= K && ldc >= N &&
+                  work_elems >= 0,
+              "smx_gemm_nt_f64_split: bad arguments");
+  return gemm_nt<double>(as_stream(stream), M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, 0, work,
+                         work_elems);
 }
 
 int smx_spd_inverse_f64(double* A, int64_t n, double* work, void* stream) {

@@ -44,7 +44,9 @@ KNOB_GRAM_WAVES = 1
 KNOB_INV_SYRKX_MIN = 2
 KNOB_GEMM64_SPLIT_TARGET = 3
 KNOB_ADMM_LOADS = 4
+KNOB_GEMM64_KERNEL = 5
 GRAM_WAVES = (4, 8)
+GEMM64_KERNELS = (1, 2)   # 1 thin kernel, 2 tiled kernel with split-K
 SYRKX_MIN = (4096, 8192, 16384, 32768)
 SPLIT_TARGETS = (256, 512, 1024, 2048, 4096)
 
@@ -163,8 +165,10 @@ def _solve_stages(n, m, K, lib, scale, Y, ws, W):
 def autotune_factor(n=20000, m=17288, K=100, seed=0, apply=True, repeats=2):
     """Time the Gram, inverse and FP64-GEMM knobs on a one-iteration solve of
     shape (n, m, K); keep the fastest value of each (one knob at a time, the
-    others at their current values).  Returns {'gram_waves': best,
-    'syrkx_min': best, 'split_target': best, 'ms': {knob name: {value: ms}}}."""
+    others at their current values; the tiled FP64 GEMM's split target only
+    when the tiled kernel wins).  Returns {'gram_waves': best, 'syrkx_min':
+    best, 'gemm64_kernel': best, 'split_target': best, 'ms': {knob name:
+    {value: ms}}}."""
     L = _lib.lib()
     g = torch.Generator(device='cuda')
     g.manual_seed(seed)
@@ -181,12 +185,16 @@ def autotune_factor(n=20000, m=17288, K=100, seed=0, apply=True, repeats=2):
              'syrkx_min': (KNOB_INV_SYRKX_MIN,
                            tuple(sorted({v for v in SYRKX_MIN if v <= n} | {max(n + 1, 256)})),
                            ('factor',)),
+             'gemm64_kernel': (KNOB_GEMM64_KERNEL, GEMM64_KERNELS, ('rhs', 'admm')),
              'split_target': (KNOB_GEMM64_SPLIT_TARGET, SPLIT_TARGETS, ('rhs', 'admm'))}
     results, best = {}, {}
     try:
         _solve_stages(n, m, K, lib, scale, Y, ws, W)          # warm-up
         for name, (knob, values, stages) in knobs.items():
             before = get_knob(knob)
+            if name == 'split_target' and best.get('gemm64_kernel') == 1:
+                best[name] = before                           # the thin kernel has no split target
+                continue
             res = {}
             try:
                 for v in values:
@@ -229,5 +237,6 @@ def ensure_tuned(n, m, K, **kw):
         f = res['factor']
         set_knob(KNOB_GRAM_WAVES, f['gram_waves'])
         set_knob(KNOB_INV_SYRKX_MIN, f['syrkx_min'])
+        set_knob(KNOB_GEMM64_KERNEL, f['gemm64_kernel'])
         set_knob(KNOB_GEMM64_SPLIT_TARGET, f['split_target'])
     return _RESULTS[key]

@@ -33,6 +33,17 @@ def test_struct_layouts_match_header():
     assert _lib.SmxNnlsProblem.w_init.offset == 5 * 8
 
 
+def test_split_gemm_refuses_bad_arguments_without_gpu():
+    """smx_gemm_nt_f64_split checks its arguments before touching the device:
+    no workspace, a leading dimension below K."""
+    from schwarmax_amd import _lib
+    L = _lib.load_library()
+    fake = ctypes.c_void_p(256)                     # never dereferenced: refused first
+    assert L.smx_gemm_nt_f64_split(4, 8, 16, 1.0, fake, 16, fake, 16, fake, 8, None, 0, None) == 2
+    assert 'smx_gemm_nt_f64_split' in _lib.last_error()
+    assert L.smx_gemm_nt_f64_split(4, 8, 16, 1.0, fake, 15, fake, 16, fake, 8, fake, 64, None) == 2
+
+
 def test_grouped_nnls_refuses_bad_arguments_without_gpu():
     """smx_nnls_admm_group checks its arguments before touching the device."""
     from schwarmax_amd import _lib
@@ -141,11 +152,11 @@ def test_tuning_knobs_without_gpu():
     v = ctypes.c_int64(0)
     defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 4,
                 tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048,
-                tune.KNOB_ADMM_LOADS: 0}
+                tune.KNOB_ADMM_LOADS: 0, tune.KNOB_GEMM64_KERNEL: 1}
     bad = {tune.KNOB_ADMM_OCCUPANCY: 9, tune.KNOB_GRAM_WAVES: 6, tune.KNOB_INV_SYRKX_MIN: 100,
-           tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3}
+           tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3, tune.KNOB_GEMM64_KERNEL: 3}
     good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 8, tune.KNOB_INV_SYRKX_MIN: 4096,
-            tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2}
+            tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2, tune.KNOB_GEMM64_KERNEL: 2}
     for k in defaults:
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0
         before = v.value
@@ -155,7 +166,7 @@ def test_tuning_knobs_without_gpu():
         assert L.smx_tune_set(k, 0) == 0
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0 and v.value == defaults[k]
         assert L.smx_tune_set(k, before) == 0
-    assert L.smx_tune_set(5, 1) == 2 and L.smx_tune_get(-1, ctypes.byref(v)) == 2
+    assert L.smx_tune_set(len(defaults), 1) == 2 and L.smx_tune_get(-1, ctypes.byref(v)) == 2
     # the occupancy entry point is the same knob
     o = ctypes.c_int32(0)
     assert L.smx_admm_set_occupancy(5) == 0 and L.smx_admm_get_occupancy(ctypes.byref(o)) == 0

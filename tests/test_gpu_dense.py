@@ -34,6 +34,38 @@ def test_gemm_nt(T, dtype, shape):
     assert np.abs(Cd.cpu().numpy().astype(np.float64) - ref).max() < tol
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("shape", [(100, 2000, 1700, 1702), (6, 1100, 1100, 1100), (130, 1201, 516, 518),
+                                   (1, 65, 14, 16), (896, 130, 300, 300), (200, 3000, 2998, 3000),
+                                   (17, 64, 16, 16), (5, 40, 33, 35)])
+def test_gemm_nt_f64_split(T, kernel, shape):
+    """C = -1.3 A B^T with a split workspace (the NNLS's c = -Y U and anchor
+    products) on the thin kernel (1: row passes, reduction tails, columns past
+    a 64-column group) and on the tiled split-K kernel (2), against FP64.  The
+    odd leading dimension of the last shape sends it to the tiled kernel in
+    both cases."""
+    from schwarmax_amd import _lib, tune
+    M, N, K, ldb = shape
+    rng = np.random.default_rng(M + 7 * N + K)
+    A = rng.normal(size=(M, K))
+    B = rng.normal(size=(N, ldb))
+    ref = -1.3 * A @ B[:, :K].T
+    Ad, Bd = T.tensor(A, device='cuda'), T.tensor(B, device='cuda')
+    Cd = T.full((M, N), float('nan'), dtype=T.float64, device='cuda')
+    work = T.empty(16 * M * N, dtype=T.float64, device='cuda')
+    before = tune.get_knob(tune.KNOB_GEMM64_KERNEL)
+    try:
+        tune.set_knob(tune.KNOB_GEMM64_KERNEL, kernel)
+        _lib.check(_lib.lib().smx_gemm_nt_f64_split(M, N, K, -1.3, _lib.ptr(Ad), K, _lib.ptr(Bd), ldb,
+                                                    _lib.ptr(Cd), N, _lib.ptr(work), work.numel(),
+                                                    _lib.stream_ptr()), 'gemm_nt_f64_split')
+    finally:
+        tune.set_knob(tune.KNOB_GEMM64_KERNEL, before)
+    got = Cd.cpu().numpy()
+    assert np.isfinite(got).all()
+    assert np.abs(got - ref).max() < 1e-13 * np.sqrt(K) * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("n", [60, 64, 300, 777, 2100])
 def test_spd_inverse(T, n):
     """FP64 inverse of an orbit-library-like Gram (+ rho I), ill-coupled blocks."""

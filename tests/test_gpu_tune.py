@@ -52,33 +52,43 @@ def test_autotune_factor_knobs():
     from schwarmax_amd import tune, nnls as GN
     from oracle import nnls as ON
     from test_gpu_nnls import _problem
-    knobs = (tune.KNOB_GRAM_WAVES, tune.KNOB_INV_SYRKX_MIN, tune.KNOB_GEMM64_SPLIT_TARGET)
+    knobs = (tune.KNOB_GRAM_WAVES, tune.KNOB_INV_SYRKX_MIN, tune.KNOB_GEMM64_SPLIT_TARGET,
+             tune.KNOB_GEMM64_KERNEL)
     before = [tune.get_knob(k) for k in knobs]
     try:
         res = tune.autotune_factor(n=3000, m=1500, K=20, repeats=1)
         print(res)
         assert set(res['ms']['gram_waves']) == set(tune.GRAM_WAVES)
-        assert set(res['ms']['split_target']) == set(tune.SPLIT_TARGETS)
+        assert set(res['ms']['gemm64_kernel']) == set(tune.GEMM64_KERNELS)
+        if res['gemm64_kernel'] == 2:                  # the tiled kernel's split target is tuned
+            assert set(res['ms']['split_target']) == set(tune.SPLIT_TARGETS)
+        else:
+            assert 'split_target' not in res['ms']
         assert 3001 in res['ms']['syrkx_min']          # every node on dgemm
         assert all(t > 0 for r in res['ms'].values() for t in r.values())
         assert tune.get_knob(tune.KNOB_GRAM_WAVES) == res['gram_waves']
         assert tune.get_knob(tune.KNOB_INV_SYRKX_MIN) == res['syrkx_min']
+        assert tune.get_knob(tune.KNOB_GEMM64_KERNEL) == res['gemm64_kernel']
         assert tune.get_knob(tune.KNOB_GEMM64_SPLIT_TARGET) == res['split_target']
         A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy, sh = _problem(3, n=300, K=12)
         w5 = np.sqrt(5.0)
         ref = ON.solve_nnls_admm_bootstrap(A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy,
                                            sh, lambda_reg=1.0, maxiter=200, w_rzphi=w5, w_xy=w5,
                                            w_h=1.0)
-        # the 8-wave Gram and the syrkx updates at every node, against the oracle
+        # the 8-wave Gram and the syrkx updates at every node, with either
+        # FP64 GEMM kernel, against the oracle
         for k, v in ((tune.KNOB_GRAM_WAVES, 8), (tune.KNOB_INV_SYRKX_MIN, 256),
                      (tune.KNOB_GEMM64_SPLIT_TARGET, 4096)):
             tune.set_knob(k, v)
-        got = GN.solve_nnls_admm_bootstrap(A_R, A_xy, *A_h, y_R, y_xy, gamma, yb_xy, *yb_h, sR, sxy,
-                                           *sh, lambda_reg=1.0, maxiter=200, w_rzphi=w5, w_xy=w5,
-                                           w_h=1.0).cpu().numpy()
-        err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        print(f"weights with 8-wave Gram, syrkx everywhere, split target 4096: {err.max():.2e}")
-        assert err.max() < 1e-4, err
+        for kern in tune.GEMM64_KERNELS:
+            tune.set_knob(tune.KNOB_GEMM64_KERNEL, kern)
+            got = GN.solve_nnls_admm_bootstrap(A_R, A_xy, *A_h, y_R, y_xy, gamma, yb_xy, *yb_h, sR,
+                                               sxy, *sh, lambda_reg=1.0, maxiter=200, w_rzphi=w5,
+                                               w_xy=w5, w_h=1.0).cpu().numpy()
+            err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            print(f"weights with 8-wave Gram, syrkx everywhere, FP64 GEMM kernel {kern}: "
+                  f"{err.max():.2e}")
+            assert err.max() < 1e-4, err
     finally:
         for k, v in zip(knobs, before):
             tune.set_knob(k, v)

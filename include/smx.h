@@ -337,8 +337,9 @@ int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
  * restores a knob's default).  None changes what a kernel computes; the
  * split-K knobs change the order in which partial sums are added.
  *   SMX_KNOB_ADMM_OCCUPANCY       1..8, default 2: as smx_admm_set_occupancy.
- *   SMX_KNOB_GRAM_WAVES           4 or 8, default 4: waves per Gram workgroup
- *                                 (128 x 64 or 128 x 32 outputs per wave).
+ *   SMX_KNOB_GRAM_WAVES           4 or 8, default 8: waves per Gram workgroup
+ *                                 (128 x 64 or 128 x 32 outputs per wave; two
+ *                                 workgroups per CU either way).
  *   SMX_KNOB_INV_SYRKX_MIN        >= 256, default 16384: nodes of the FP64
  *                                 inverse with at least this many rows form
  *                                 their symmetric updates with rocBLAS syrkx,

@@ -1231,16 +1231,18 @@ __global__ void fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
 // chunk c) = rows 128 I + r, indices 32 c + q at ((I nchk + c) 128 + r) 32 + q.
 // ---------------------------------------------------------------------------
 constexpr int GR_TM = 128;   // rows (A) per tile; the B operand is two such blocks
-// Waves per Gram workgroup (the tuning knob KN_GRAM_WAVES: 4 or 8).  Unlike
-// the ADMM product, the Gram does three MFMAs per fragment pair and every
-// wave reads all 128 A rows from LDS: 8 waves of 128 x 32 outputs took
-// 28.4 ms at plain-20k against 18.1 ms for 4 waves of 128 x 64.
+// Waves per Gram workgroup (the tuning knob KN_GRAM_WAVES: 4 or 8), two
+// workgroups per CU either way (61 KB of LDS each).  8 waves of 128 x 32
+// outputs (114 VGPRs) read all 128 A rows from LDS per wave, twice the LDS
+// reads of 4 waves of 128 x 64 (242 VGPRs), but run four waves per SIMD
+// instead of two: at plain-20k 25.8 against 29.8 ms, at n = 40,000,
+// m = 35,600 174.5 against 192.0 ms.
 template <int WAVES>
 struct GramShape {
   static constexpr int NT = 64 * WAVES;
   static constexpr int WCB = 2 * GR_TM / (32 * WAVES);   // 32-column blocks per wave
   static constexpr int PASSES = GR_TM / (NT / 4);         // staging passes per 128-row tile
-  static constexpr int OCC = WAVES == 4 ? AG_OCC : 1;     // workgroups per CU (registers)
+  static constexpr int OCC = 2;                            // workgroups per CU (registers, LDS)
   static_assert(WCB >= 1 && PASSES >= 1, "Gram wave layout");
 };
 
@@ -1310,25 +1312,29 @@ static int64_t gram_tiles(int64_t gpad) {
 static int64_t gram_grid(int64_t gpad) { return 8 * ((gram_tiles(gpad) + 7) / 8); }
 
 // Segments of the Gram's reduction accumulated in FP32: each of GR_FLUSH
-// launches covers 1/GR_FLUSH of the m products of every entry and adds its
-// FP32 sums into the FP64 Q (the first stores them).  One FP32 accumulation
-// over all m = 17,288 design rows of plain-20k moves the NNLS weights by
-// 1.7e-4 from an FP64 Gram (the library's conditioning amplifies it); 8
-// segments keep that near 1e-5 for about two extra reads and writes of
-// the lower triangle of Q.
+// segments covers 1/GR_FLUSH of the m products of every entry, and the
+// workgroup adds its FP32 sums into the FP64 Q at the end of the segment
+// (the first segment stores them).  One FP32 accumulation over all
+// m = 17,288 design rows of plain-20k moves the NNLS weights by 1.7e-4 from
+// an FP64 Gram (the library's conditioning amplifies it); 8 segments keep
+// that near 1e-5 for about two extra reads and writes of the lower triangle
+// of Q.  A workgroup runs all segments of its tile, so the read-modify-writes
+// of Q overlap other workgroups' MFMAs (one launch per segment: 26.5 ms at
+// plain-20k).
 #ifndef GR_FLUSH
 #define GR_FLUSH 8
 #endif
 
 // One (128 x 256) tile of Q per workgroup (WAVES waves of 128 x 256/WAVES),
-// 32-index chunks ch0 .. ch1 - 1 staged through LDS, hi*hi + hi*lo + lo*hi
-// per 16-deep step; stores (first) or adds the segment's sums to Q[i][j] for
+// 32-index chunks staged through LDS, hi*hi + hi*lo + lo*hi per 16-deep
+// step; the nchk_total chunks in segments of seg_chunks chunks, after each of which the
+// segment's sums are stored (first segment) or added to Q[i][j] for
 // j <= i < n.
 template <int WAVES>
 __global__ __launch_bounds__(GramShape<WAVES>::NT, GramShape<WAVES>::OCC) void gram_split_kernel(
     int64_t n, int64_t gpad, int64_t mpad, const _Float16* __restrict__ Uh,
     const _Float16* __restrict__ Ul, const float* __restrict__ us, double* __restrict__ Q,
-    int64_t ch0, int64_t ch1, int first) {
+    int64_t seg_chunks, int64_t nchk_total) {
   constexpr int GR_NT = GramShape<WAVES>::NT;
   constexpr int GR_WCB = GramShape<WAVES>::WCB;
   constexpr int GR_PASSES = GramShape<WAVES>::PASSES;
@@ -1349,6 +1355,9 @@ __global__ __launch_bounds__(GramShape<WAVES>::NT, GramShape<WAVES>::OCC) void g
   const int64_t tileA = I * nchk, tileB0 = 2 * J * nchk, tileB1 = (2 * J + 1) * nchk;
   const int r0 = tid >> 2, seg = 8 * (tid & 3);   // rows r0 + (GR_NT / 4) h of a 128-row tile
   f32x16 acc[4][GR_WCB];
+  for (int64_t ch0 = 0; ch0 < nchk_total; ch0 += seg_chunks) {
+  const int64_t ch1 = ch0 + seg_chunks < nchk_total ? ch0 + seg_chunks : nchk_total;
+  const bool first = ch0 == 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -1394,23 +1403,33 @@ __global__ __launch_bounds__(GramShape<WAVES>::NT, GramShape<WAVES>::OCC) void g
     }
     __syncthreads();
   }
+  // The scales and addresses of the flush are the same in every segment; the
+  // compiler would hoist them out of the segment loop and keep them (64 FP64
+  // scales, addresses and masks per lane) live across the chunk loop,
+  // spilling.  Laundering their inputs once per segment keeps them local.
+  const float* usg = us;
+  double* Qg = Q;
+  int64_t i0g = i0, j0g = j0;
+  int gg = g, cg = c;
+  asm volatile("" : "+s"(usg), "+s"(Qg), "+s"(i0g), "+s"(j0g), "+v"(gg), "+v"(cg));
   // 32x32 accumulator layout: lane (g, c), element r -> row 8(r/4) + 4g + r%4, column c
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
     for (int cb = 0; cb < GR_WCB; ++cb) {
-      const int64_t j = j0 + wave * (2 * GR_TM / GR_WAVES) + cb * 32 + c;
-      const double sj = (j < n) ? static_cast<double>(us[j]) : 1.0;
+      const int64_t j = j0g + wave * (2 * GR_TM / GR_WAVES) + cb * 32 + cg;
+      const double sj = (j < n) ? static_cast<double>(usg[j]) : 1.0;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t i = i0 + rb * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+        const int64_t i = i0g + rb * 32 + 8 * (r >> 2) + 4 * gg + (r & 3);
         if (i < n && j <= i) {
-          const double v = static_cast<double>(acc[rb][cb][r]) / (static_cast<double>(us[i]) * sj);
-          double* q = Q + i * n + j;
+          const double v = static_cast<double>(acc[rb][cb][r]) / (static_cast<double>(usg[i]) * sj);
+          double* q = Qg + i * n + j;
           *q = first ? v : *q + v;
         }
       }
     }
+  }   // segments
 }
 
 
@@ -1679,17 +1698,12 @@ static int form_gram64(hipStream_t st, const NnlsWs& w, const double* lib, int64
   SMX_LAUNCHED();
   const int64_t nchk = w.mpad / AG_KP;
   const int64_t seg = (nchk + GR_FLUSH - 1) / GR_FLUSH;
-  for (int64_t ch0 = 0; ch0 < nchk; ch0 += seg) {
-    const int64_t ch1 = ch0 + seg < nchk ? ch0 + seg : nchk;
-    if (knob(KN_GRAM_WAVES) == 8)
-      hipLaunchKernelGGL(gram_split_kernel<8>, dim3((unsigned)gram_grid(w.gpad)),
-                         dim3(GramShape<8>::NT), 0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64,
-                         ch0, ch1, ch0 == 0 ? 1 : 0);
-    else
-      hipLaunchKernelGGL(gram_split_kernel<4>, dim3((unsigned)gram_grid(w.gpad)),
-                         dim3(GramShape<4>::NT), 0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64,
-                         ch0, ch1, ch0 == 0 ? 1 : 0);
-  }
+  if (knob(KN_GRAM_WAVES) == 8)
+    hipLaunchKernelGGL(gram_split_kernel<8>, dim3((unsigned)gram_grid(w.gpad)), dim3(GramShape<8>::NT),
+                       0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64, seg, nchk);
+  else
+    hipLaunchKernelGGL(gram_split_kernel<4>, dim3((unsigned)gram_grid(w.gpad)), dim3(GramShape<4>::NT),
+                       0, st, n, w.gpad, w.mpad, w.Uh, w.Ul, w.us, w.Q64, seg, nchk);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(regularise64_kernel, dim3(1), dim3(1024), 0, st, w.Q64, n,
                      lambda_reg / static_cast<double>(n), w.rho64, w.rho);

@@ -8,10 +8,11 @@ namespace smx {
 
 namespace {
 // Defaults, as measured at the plain 20,000-orbit and the 40,000-orbit NNLS
-// shapes: occupancy 2, 4-wave Gram, syrkx from 16,384 rows, and a split-K
-// target of 2,048 workgroups for the thin FP64 GEMMs (the right-hand sides
-// and one anchor: 5.9 ms against 6.5 ms at 1,024 at plain-20k shapes).
-constexpr int64_t kDefault[KN_COUNT] = {2, 4, 16384, 2048, 0, 1};
+// shapes: occupancy 2, 8-wave Gram (25.8 against 29.8 ms for 4 waves at
+// plain-20k shapes), syrkx from 16,384 rows, the thin FP64 GEMM, and a
+// split-K target of 2,048 workgroups for the tiled FP64 GEMM (the right-hand
+// sides and one anchor: 5.9 ms against 6.5 ms at 1,024 at plain-20k shapes).
+constexpr int64_t kDefault[KN_COUNT] = {2, 8, 16384, 2048, 0, 1};
 std::atomic<int64_t> g_knob[KN_COUNT] = {{kDefault[0]}, {kDefault[1]}, {kDefault[2]},
                                          {kDefault[3]}, {kDefault[4]}, {kDefault[5]}};
 

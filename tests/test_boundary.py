@@ -150,12 +150,12 @@ def test_tuning_knobs_without_gpu():
     from schwarmax_amd import _lib, tune
     L = _lib.load_library()
     v = ctypes.c_int64(0)
-    defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 4,
+    defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 8,
                 tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048,
                 tune.KNOB_ADMM_LOADS: 0, tune.KNOB_GEMM64_KERNEL: 1}
     bad = {tune.KNOB_ADMM_OCCUPANCY: 9, tune.KNOB_GRAM_WAVES: 6, tune.KNOB_INV_SYRKX_MIN: 100,
            tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3, tune.KNOB_GEMM64_KERNEL: 3}
-    good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 8, tune.KNOB_INV_SYRKX_MIN: 4096,
+    good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 4, tune.KNOB_INV_SYRKX_MIN: 4096,
             tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2, tune.KNOB_GEMM64_KERNEL: 2}
     for k in defaults:
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0

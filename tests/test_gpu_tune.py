@@ -75,9 +75,9 @@ def test_autotune_factor_knobs():
         ref = ON.solve_nnls_admm_bootstrap(A_R, A_xy, A_h, y_R, y_xy, gamma, yb_xy, yb_h, sR, sxy,
                                            sh, lambda_reg=1.0, maxiter=200, w_rzphi=w5, w_xy=w5,
                                            w_h=1.0)
-        # the 8-wave Gram and the syrkx updates at every node, with either
+        # the 4-wave Gram and the syrkx updates at every node, with either
         # FP64 GEMM kernel, against the oracle
-        for k, v in ((tune.KNOB_GRAM_WAVES, 8), (tune.KNOB_INV_SYRKX_MIN, 256),
+        for k, v in ((tune.KNOB_GRAM_WAVES, 4), (tune.KNOB_INV_SYRKX_MIN, 256),
                      (tune.KNOB_GEMM64_SPLIT_TARGET, 4096)):
             tune.set_knob(k, v)
         for kern in tune.GEMM64_KERNELS:
@@ -86,7 +86,7 @@ def test_autotune_factor_knobs():
                                                sxy, *sh, lambda_reg=1.0, maxiter=200, w_rzphi=w5,
                                                w_xy=w5, w_h=1.0).cpu().numpy()
             err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-            print(f"weights with 8-wave Gram, syrkx everywhere, FP64 GEMM kernel {kern}: "
+            print(f"weights with 4-wave Gram, syrkx everywhere, FP64 GEMM kernel {kern}: "
                   f"{err.max():.2e}")
             assert err.max() < 1e-4, err
     finally:

@@ -412,20 +412,23 @@ __device__ __forceinline__ float pow2_scale_for(float amax, int target_exp) {
 // jb = j / 32, cc = j % 32, c = i / 32, t = i % 32 / 16, g = i % 16 / 8,
 // e = i % 8 (nch = npad / 32): one (column block, chunk, step) is a
 // contiguous 1 KB run in lane order.
-__device__ __forceinline__ int64_t m_frag_index(int64_t j, int64_t i, int64_t nch) {
-  const int64_t r = i % AG_KP;
-  return ((((j / 32) * nch + i / AG_KP) * 2 + r / 16) * 64 + 32 * ((r % 16) / 8) + j % 32) * 8 +
-         r % 8;
-}
 
 // M (n, n) FP64 -> scaled fp16 hi / lo (npad x npad, fragment-blocked), zero
-// padded.  One thread per 8 consecutive i of a row (one 16-byte piece).
+// padded.  One thread per 16-byte piece of the planes, in their order: piece
+// p = ((jb nch + c) 2 + t) 64 + 32 g + cc holds column j = 32 jb + cc at
+// indices i0 = 32 c + 16 t + 8 g .. i0 + 7 (elements 8 p .. 8 p + 7).  A
+// wave writes 1 KB runs of each plane and reads the 128-byte lines of 32
+// rows (lanes cc and 32 + cc share a line); with one thread per 8
+// consecutive i of a row instead, the reads were contiguous but each wave's
+// writes were 64 scattered 16-byte pieces (2.2 ms at n = 20,000).
 __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t npad,
                                const float* __restrict__ rho_p, _Float16* __restrict__ Mh,
                                _Float16* __restrict__ Ml) {
-  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= npad * npad / 8) return;
-  const int64_t j = idx / (npad / 8), i0 = (idx - j * (npad / 8)) * 8;
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= npad * npad / 8) return;
+  const int64_t nch = npad / AG_KP;
+  const int64_t cc = p % 32, g = (p / 32) % 2, t = (p / 64) % 2, rest = p / 128;
+  const int64_t j = 32 * (rest / nch) + cc, i0 = AG_KP * (rest % nch) + 16 * t + 8 * g;
   const double sc = pow2_scale_for(1.f / *rho_p, 13);   // power of two; |M_ij| <= 1/rho
   half8 h, l;
 #pragma unroll
@@ -435,9 +438,8 @@ __global__ void split_m_kernel(const double* __restrict__ M, int64_t n, int64_t 
     h[e] = static_cast<_Float16>(v);
     l[e] = static_cast<_Float16>(v - static_cast<double>(h[e]));
   }
-  const int64_t d = m_frag_index(j, i0, npad / AG_KP);
-  *reinterpret_cast<half8*>(Mh + d) = h;
-  *reinterpret_cast<half8*>(Ml + d) = l;
+  *reinterpret_cast<half8*>(Mh + 8 * p) = h;
+  *reinterpret_cast<half8*>(Ml + 8 * p) = l;
 }
 
 // R operand staging.  Blocked FP32 layout of R (written by the update

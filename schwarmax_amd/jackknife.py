@@ -203,15 +203,25 @@ def model_deltaChi2_jackknife(density_func_, potential_func_, chi2_func_, params
             acc += (dl ** k)[:, None] * s
         return acc
 
+    # the Woodbury correction and the z / u update of each iteration, with the
+    # next right-hand sides, in one kernel (one workgroup per replicate)
+    L = _lib.lib()
+    V = V.contiguous()
+    Pg = Pg.contiguous()
+    Cinv = Cinv.contiguous()
+    rho_g = rho_g.contiguous()
+    c_g = c_g.contiguous()
+    r = rho_g[:, None] * (z - u) - c_g
     for it in range(int(nnls_maxiter)):
-        r = rho_g[:, None] * (z - u) - c_g
-        x = shifted_inverse_iter(r, delta, it == int(nnls_maxiter) - 1)
-        tt = torch.einsum('gde,ge->gd', Cinv, torch.einsum('gdn,gn->gd', V, x))
-        w = x + torch.einsum('gdn,gd->gn', Pg, tt)
-        w_hat = alpha * w + (1.0 - alpha) * z
-        z_new = torch.clamp(w_hat + u, min=0.0)
-        u = u + w_hat - z_new
-        z = z_new
+        x = shifted_inverse_iter(r, delta, it == int(nnls_maxiter) - 1).contiguous()
+        _lib.check(L.smx_jackknife_admm_step(G, n, d, _lib.ptr(x, torch.float64),
+                                             _lib.ptr(V, torch.float64), _lib.ptr(Pg, torch.float64),
+                                             _lib.ptr(Cinv, torch.float64),
+                                             _lib.ptr(rho_g, torch.float64),
+                                             _lib.ptr(c_g, torch.float64), alpha,
+                                             _lib.ptr(z, torch.float64), _lib.ptr(u, torch.float64),
+                                             _lib.ptr(r, torch.float64), _lib.stream_ptr()),
+                   'smx_jackknife_admm_step')
 
     # per-replicate model and chi^2 (model.py:531-577)
     d3, d2, h1, h2, h3, h4, _, _ = MD.compute_model_single(z, lib, P, B, gk, t['v0_ap'], t['s_ap'])

@@ -246,17 +246,18 @@ int smx_nnls_operator_apply_hi(int64_t n, int64_t m, int64_t K_max, void* worksp
                                const float* D, int64_t K, float* out, void* stream);
 /* One ADMM iteration of all G jackknife replicates after the operator
  * products (model.py:490-577 re-solves nnls.py:51-66 per replicate): with
- * x (G, n) the shifted operator applied to this iteration's right-hand sides,
- * V (G, d, n) the design rows each replicate drops, P (G, d, n) the shifted
- * operator applied to them and Cinv (G, d, d) the inverse Woodbury capacitance
- * matrices, w = x + P^T Cinv (V x); then, in place, w_hat = alpha w +
- * (1 - alpha) z, z = max(w_hat + u, 0), u = u + w_hat - z (nnls.py:155-162),
- * and r (G, n) = rho_g (z - u) - c (G, n), the next right-hand sides.
- * FP64 throughout; d <= 2048. */
-int smx_jackknife_admm_step(int64_t G, int64_t n, int64_t d, const double* x, const double* V,
-                            const double* P, const double* Cinv, const double* rho_g,
-                            const double* c, double alpha, double* z, double* u, double* r,
-                            void* stream);
+ * S (T, G, n) FP32 the products r M^(k+1) of this iteration's right-hand
+ * sides r, x = sum_k delta_g^k S_k in FP64 (the shifted operator by its
+ * Neumann series), V (G, d, n) the design rows each replicate drops,
+ * P (G, d, n) the shifted operator applied to them and Cinv (G, d, d) the
+ * inverse Woodbury capacitance matrices, w = x + P^T Cinv (V x); then, in
+ * place, w_hat = alpha w + (1 - alpha) z, z = max(w_hat + u, 0),
+ * u = u + w_hat - z (nnls.py:155-162), and r (G, n, FP32) = rho_g (z - u) - c
+ * (G, n), the next right-hand sides.  d <= 2048, T <= 64. */
+int smx_jackknife_admm_step(int64_t G, int64_t n, int64_t d, int64_t T, const float* S,
+                            const double* delta, const double* V, const double* P,
+                            const double* Cinv, const double* rho_g, const double* c, double alpha,
+                            double* z, double* u, float* r, void* stream);
 
 /* ---- batched evaluation (many theta, one launch per stage) -------------- */
 /* Workspace for smx_build_library_batch: the persistent grid's slab pool plus

@@ -108,8 +108,8 @@ _SIGS = [
     ('smx_nnls_operator', ctypes.c_int, [_P, _I64, _I64, _I64, _P, ctypes.c_double, _I64, _P, _P, _P]),
     ('smx_nnls_operator_apply', ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, _P, _P]),
     ('smx_nnls_operator_apply_hi', ctypes.c_int, [_I64, _I64, _I64, _P, _P, _I64, _P, _P]),
-    ('smx_jackknife_admm_step', ctypes.c_int, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _D, _P, _P,
-                                                _P, _P]),
+    ('smx_jackknife_admm_step', ctypes.c_int, [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                                _D, _P, _P, _P, _P]),
     ('smx_cylspline_phi_m', ctypes.c_int, [_I32, _P, _I32, _P, _I32, _P, _P, _P, _I64, _P, _D, _P, _P]),
     ('smx_cylspline_eval', ctypes.c_int, [ctypes.POINTER(SmxCylspline), _P, _I64, _P, _P, _P, _P]),
     ('smx_legendre_q_half', ctypes.c_int, [_I32, _P, _I64, _P, _P]),

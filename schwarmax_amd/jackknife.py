@@ -50,47 +50,61 @@ class _Operator:
                                        _lib.stream_ptr()), 'smx_nnls_operator')
         self.rho = float(rho.item())
 
-    def apply(self, R, hi_only=False):
-        """R (K, n) -> R M (FP32 in, FP64 out), K <= K_max per call; hi_only:
-        R_hi M_hi (the delta product)."""
+    def apply_into(self, R32, out, hi_only=False):
+        """out (K, n) <- R32 M, both FP32 and contiguous, K <= K_max per call;
+        hi_only: R_hi M_hi (the delta product)."""
         L = _lib.lib()
         fn = L.smx_nnls_operator_apply_hi if hi_only else L.smx_nnls_operator_apply
+        for k0 in range(0, R32.shape[0], self.K_max):
+            k1 = min(k0 + self.K_max, R32.shape[0])
+            _lib.check(fn(self.n, self.m, self.K_max, _lib.ptr(self.ws),
+                          _lib.ptr(R32[k0:k1], torch.float32), k1 - k0,
+                          _lib.ptr(out[k0:k1], torch.float32), _lib.stream_ptr()),
+                       'smx_nnls_operator_apply')
+        return out
+
+    def apply(self, R, hi_only=False):
+        """R (K, n) -> R M (FP32 in, FP64 out)."""
         R = R.to(torch.float32).contiguous()
-        out = torch.empty_like(R)
-        for k0 in range(0, R.shape[0], self.K_max):
-            k1 = min(k0 + self.K_max, R.shape[0])
-            _lib.check(fn(self.n, self.m, self.K_max, _lib.ptr(self.ws), _lib.ptr(R[k0:k1]), k1 - k0,
-                          _lib.ptr(out[k0:k1]), _lib.stream_ptr()), 'smx_nnls_operator_apply')
-        return out.double()
+        return self.apply_into(R, torch.empty_like(R), hi_only).double()
 
 
 class _DeltaApply:
     """Products R_t M of a slowly changing input sequence R_t (one per ADMM
-    iteration): a full product when the input has moved by more than TAU of
-    some row's size since the last full one, otherwise that product plus the
-    delta product (R_t - R_b)_hi M_hi, as the solver's own ADMM loop does
-    (one host check per product)."""
+    iteration), into a fixed FP32 output: a full product when the input has
+    moved by more than TAU of some row's size since the last full one,
+    otherwise that product plus the delta product (R_t - R_b)_hi M_hi, as the
+    solver's own ADMM loop does (one host check per product)."""
 
     TAU = 2.0 ** -10
 
-    def __init__(self, op):
+    def __init__(self, op, out):
         self.op = op
-        self.base = None
+        self.out = out          # (K, n) FP32
+        self.base_R = None      # the input of the last full product, and that product
+        self.base_Y = None
         self.full = 0
         self.calls = 0
 
-    def __call__(self, R):
+    def __call__(self, R32, force_full=False):
+        """self.out <- R32 M (R32: FP32, contiguous, left unchanged)."""
         self.calls += 1
-        R32 = R.to(torch.float32).contiguous()
-        if self.base is not None:
-            D = R32 - self.base[0]
-            moved = (D.abs().amax(dim=1) > self.TAU * R32.abs().amax(dim=1)).any()
-            if not bool(moved):
-                return self.base[1] + self.op.apply(D, hi_only=True)
-        Y = self.op.apply(R32)
-        self.base = (R32, Y)
+        if self.base_R is not None and not force_full:
+            D = R32 - self.base_R
+            dmax = torch.linalg.vector_norm(D, ord=float('inf'), dim=1)
+            rmax = torch.linalg.vector_norm(R32, ord=float('inf'), dim=1)
+            if not bool((dmax > self.TAU * rmax).any()):
+                self.op.apply_into(D, self.out, hi_only=True)
+                self.out += self.base_Y
+                return self.out
+        self.op.apply_into(R32, self.out)
+        if self.base_R is None:
+            self.base_R, self.base_Y = R32.clone(), self.out.clone()
+        else:
+            self.base_R.copy_(R32)
+            self.base_Y.copy_(self.out)
         self.full += 1
-        return Y
+        return self.out
 
 
 def _neumann_terms(q, tol=1e-13, cap=40):
@@ -190,37 +204,35 @@ def model_deltaChi2_jackknife(density_func_, potential_func_, chi2_func_, params
     z = torch.full((G, n), float(torch.sum(y_R)) / n, dtype=torch.float64, device='cuda')
     u = torch.zeros_like(z)
     alpha = 1.6
-    terms = [_DeltaApply(op) for _ in range(T)]
+    # the T Neumann terms' products of one iteration, S[k] = r M^(k+1) (FP32);
+    # term k + 1 multiplies term k's output
+    S = torch.empty((T, G, n), dtype=torch.float32, device='cuda')
+    terms = [_DeltaApply(op, S[k]) for k in range(T)]
 
-    def shifted_inverse_iter(R, dl, last):
-        if last:
-            for tm in terms:
-                tm.base = None
-        s = terms[0](R)
-        acc = s.clone()
-        for k in range(1, T):
-            s = terms[k](s)
-            acc += (dl ** k)[:, None] * s
-        return acc
-
-    # the Woodbury correction and the z / u update of each iteration, with the
-    # next right-hand sides, in one kernel (one workgroup per replicate)
+    # x = sum_k delta^k S[k] (FP64), the Woodbury correction and the z / u
+    # update of each iteration, with the next right-hand sides (FP32, the
+    # products' input), in one kernel (one workgroup per replicate)
     L = _lib.lib()
     V = V.contiguous()
     Pg = Pg.contiguous()
     Cinv = Cinv.contiguous()
     rho_g = rho_g.contiguous()
     c_g = c_g.contiguous()
-    r = rho_g[:, None] * (z - u) - c_g
+    delta = delta.contiguous()
+    r32 = (rho_g[:, None] * (z - u) - c_g).to(torch.float32).contiguous()
     for it in range(int(nnls_maxiter)):
-        x = shifted_inverse_iter(r, delta, it == int(nnls_maxiter) - 1).contiguous()
-        _lib.check(L.smx_jackknife_admm_step(G, n, d, _lib.ptr(x, torch.float64),
+        last = it == int(nnls_maxiter) - 1          # the last iteration's products are full
+        R = r32
+        for tm in terms:
+            R = tm(R, force_full=last)
+        _lib.check(L.smx_jackknife_admm_step(G, n, d, T, _lib.ptr(S, torch.float32),
+                                             _lib.ptr(delta, torch.float64),
                                              _lib.ptr(V, torch.float64), _lib.ptr(Pg, torch.float64),
                                              _lib.ptr(Cinv, torch.float64),
                                              _lib.ptr(rho_g, torch.float64),
                                              _lib.ptr(c_g, torch.float64), alpha,
                                              _lib.ptr(z, torch.float64), _lib.ptr(u, torch.float64),
-                                             _lib.ptr(r, torch.float64), _lib.stream_ptr()),
+                                             _lib.ptr(r32, torch.float32), _lib.stream_ptr()),
                    'smx_jackknife_admm_step')
 
     # per-replicate model and chi^2 (model.py:531-577)

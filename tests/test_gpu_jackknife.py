@@ -50,35 +50,46 @@ def test_jackknife_matches_oracle(galaxy, n_groups, iters):
     assert abs(float(got) - ref) <= 1e-3 * abs(ref), (float(got), ref)
 
 
-@pytest.mark.parametrize("G,n,d", [(7, 1000, 5), (3, 777, 15), (2, 64, 1), (4, 300, 17)])
-def test_jackknife_admm_step_matches_torch(G, n, d):
-    """smx_jackknife_admm_step against the Woodbury correction and ADMM update
-    written out in FP64 torch (x + P^T Cinv (V x), then nnls.py:155-162 and the
-    next right-hand sides), on random inputs with d across several passes."""
+@pytest.mark.parametrize("G,n,d,T", [(7, 1000, 5, 3), (3, 777, 15, 1), (2, 64, 1, 5), (4, 300, 17, 2)])
+def test_jackknife_admm_step_matches_torch(G, n, d, T):
+    """smx_jackknife_admm_step against its formulas written out in FP64 torch:
+    x = sum_k delta^k S_k, x + P^T Cinv (V x), then nnls.py:155-162 and the
+    next right-hand sides (FP32), on random inputs with d across several
+    passes of the dot products."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from schwarmax_amd import _lib
     gen = torch.Generator(device='cuda')
-    gen.manual_seed(G * n + d)
+    gen.manual_seed(G * n + d + T)
     rnd = lambda *s: torch.randn(*s, dtype=torch.float64, device='cuda', generator=gen)  # noqa: E731
-    x, V, P, Cinv = rnd(G, n), rnd(G, d, n), rnd(G, d, n), rnd(G, d, d)
+    S = rnd(T, G, n).to(torch.float32)
+    delta = rnd(G).abs() * 0.1
+    V, P, Cinv = rnd(G, d, n), rnd(G, d, n), rnd(G, d, d)
     rho_g, c = rnd(G).abs() + 1.0, rnd(G, n)
     z, u = rnd(G, n).abs(), rnd(G, n) * 0.1
     alpha = 1.6
+    x = S[0].double().clone()
+    for k in range(1, T):
+        x += (delta ** k)[:, None] * S[k].double()
     s = torch.bmm(V, x.unsqueeze(-1)).squeeze(-1)
     w = x + torch.bmm(P.transpose(1, 2).contiguous(), torch.bmm(Cinv, s.unsqueeze(-1))).squeeze(-1)
     w_hat = alpha * w + (1.0 - alpha) * z
     z_ref = torch.clamp(w_hat + u, min=0.0)
     u_ref = u + w_hat - z_ref
     r_ref = rho_g[:, None] * (z_ref - u_ref) - c
-    r = torch.full_like(x, float('nan'))
-    _lib.check(_lib.lib().smx_jackknife_admm_step(G, n, d, _lib.ptr(x), _lib.ptr(V), _lib.ptr(P),
+    r = torch.full((G, n), float('nan'), dtype=torch.float32, device='cuda')
+    _lib.check(_lib.lib().smx_jackknife_admm_step(G, n, d, T, _lib.ptr(S, torch.float32),
+                                                  _lib.ptr(delta), _lib.ptr(V), _lib.ptr(P),
                                                   _lib.ptr(Cinv), _lib.ptr(rho_g), _lib.ptr(c), alpha,
-                                                  _lib.ptr(z), _lib.ptr(u), _lib.ptr(r),
-                                                  _lib.stream_ptr()), 'smx_jackknife_admm_step')
+                                                  _lib.ptr(z), _lib.ptr(u),
+                                                  _lib.ptr(r, torch.float32), _lib.stream_ptr()),
+               'smx_jackknife_admm_step')
     # the dot products over n and the rank-d sum round differently from bmm's
     tol = 1e-13 * (float(w.abs().max()) + float(c.abs().max())) * (d + 1) * float(rho_g.max())
-    for got, ref in ((z, z_ref), (u, u_ref), (r, r_ref)):
+    for got, ref in ((z, z_ref), (u, u_ref)):
         assert torch.isfinite(got).all()
         assert float((got - ref).abs().max()) <= tol, float((got - ref).abs().max())
+    # r is stored in FP32, the operator products' input precision
+    err = float((r.double() - r_ref).abs().max())
+    assert torch.isfinite(r).all() and err <= tol + 2.0 ** -23 * float(r_ref.abs().max()), err

@@ -164,19 +164,20 @@ __global__ __launch_bounds__(256, 1) void gemm_thin64_kernel(
 }
 
 // SIMDs of the current device (one thin-GEMM wave each).
+// Per device, queried once (devices 0..63; others are queried each call).
 static int64_t device_simds() {
-  static int simds = 0;
+  constexpr int kDevs = 64;
+  static int simds[kDevs] = {};
   static std::mutex mu;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 1024;
   std::lock_guard<std::mutex> lock(mu);
-  if (simds == 0) {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
-      simds = 4 * cus;
-    else
-      simds = 1024;
-  }
-  return simds;
+  if (dev < kDevs && simds[dev] > 0) return simds[dev];
+  int cus = 0;
+  const int v = (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                 cus > 0) ? 4 * cus : 1024;
+  if (dev < kDevs) simds[dev] = v;
+  return v;
 }
 
 // Launch shape of the thin GEMM: row passes of at most 16 TH_MAXRB rows, and

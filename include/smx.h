@@ -104,12 +104,20 @@ int smx_initial_conditions(const smx_potential* p, const double* w0, const doubl
  * aperture, which is where JAX's gather of index nX*nY clamps;
  * integrates.py:132-143, utils.py:429).  Ids outside [0, num_vbin) are
  * dropped, as segment_sum drops them. */
-/* Bytes of the per-integration accumulator workspace for n_int integrations. */
+/* Bytes of the per-integration accumulator slabs of n_int integrations: the
+ * workspace of smx_build_library_ckpt and smx_build_library_traj. */
 int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int);
-/* Integrate n_int = n*r orbits with fused binning into the workspace, then
- * average realizations into lib (n, P + 6B) laid out as
- * [Rzphi (P) | surface density (B) | h1 | h2 | h3 | h4 | h0] per column.
- * valid/T_int/t_final (n_int,) are per-integration diagnostics.
+/* Bytes of the workspace of smx_build_library for n_int integrations: the
+ * larger of the slabs and the sample rows (N_max 64-byte rows each). */
+int64_t smx_library_build_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int);
+/* Integrate n_int = n*r orbits, then bin and average realizations into lib
+ * (n, P + 6B) laid out as [Rzphi (P) | surface density (B) | h1 | h2 | h3 |
+ * h4 | h0] per column.  The integrator stores every attempt's state in the
+ * workspace and one workgroup per column bins them into LDS histograms
+ * (SMX_KNOB_LIB_BINNING = 2 selects the fused integrator, which bins into
+ * per-integration slabs; both give the same bits: the sums are 64-bit fixed
+ * point).  valid/T_int/t_final (n_int,) are per-integration diagnostics.
+ * workspace: smx_library_build_bytes(g, s, n*r) bytes.
  * Replaces _integrate_adaptive_batch_chunked_vmap (integrates.py:303-313). */
 int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                       const double* wb, const double* T_total, const double* dt0,
@@ -122,7 +130,7 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
  * loops, harnesses that forbid hipMalloc).  Same arguments and results as
  * smx_build_library without the workspace: the accumulator slabs live in a
  * fixed device array of SMX_NOALLOC_SCRATCH_BYTES that each GPU holds for the
- * life of the process, so n * r * (P rounded up to 8 + 8 B) * 8 bytes
+ * life of the process, so n * r * (P rounded up to 8 + 6 B) * 8 bytes
  * (smx_library_workspace_bytes) must fit in it; larger calls are rejected
  * (code 2, message in smx_last_error).  No hipMalloc / hipFree.  Calls that
  * overlap on one GPU share that array and race. */
@@ -374,7 +382,12 @@ int smx_admm_get_occupancy(int32_t* workgroups_per_cu);
  *                                 products with few rows (c = -Y U, the ADMM
  *                                 anchors): 1 the thin kernel (operands
  *                                 straight into registers, one wave per SIMD),
- *                                 2 the 128 x 128-tiled kernel with split-K. */
+ *                                 2 the 128 x 128-tiled kernel with split-K.
+ *   SMX_KNOB_LIB_BINNING          1 or 2, default 1: how smx_build_library
+ *                                 bins: 1 the integrator stores samples and
+ *                                 one workgroup per column bins them into LDS
+ *                                 histograms, 2 the integrator bins into
+ *                                 per-integration slabs (the same bits). */
 enum {
   SMX_KNOB_ADMM_OCCUPANCY = 0,
   SMX_KNOB_GRAM_WAVES = 1,
@@ -382,7 +395,8 @@ enum {
   SMX_KNOB_GEMM64_SPLIT_TARGET = 3,
   SMX_KNOB_ADMM_LOADS = 4,
   SMX_KNOB_GEMM64_KERNEL = 5,
-  SMX_KNOB_COUNT = 6
+  SMX_KNOB_LIB_BINNING = 6,
+  SMX_KNOB_COUNT = 7
 };
 int smx_tune_set(int32_t knob, int64_t value);
 int smx_tune_get(int32_t knob, int64_t* value);

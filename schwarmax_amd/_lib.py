@@ -86,6 +86,8 @@ _SIGS = [
     ('smx_initial_conditions', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), _P, _P, _P, _I64, _I32, _I32, _D, _D, _P, _P, _P, _P]),
     ('smx_library_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I64]),
+    ('smx_library_build_bytes', _I64,
+     [ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule), _I64]),
     ('smx_build_library', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -232,6 +234,20 @@ def bin_map(bin_mapping, nX, nY, device='cuda'):
 def stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def dispersions(s, device='cuda'):
+    """The apertures' Gauss-Hermite dispersions s as a device FP64 tensor.
+
+    The library kernels hold the binned Gauss-Hermite terms as 64-bit fixed
+    point scaled by a bound proportional to 1/s, so every s must be positive
+    and finite; the reference's gh weights (ghmoments.py:6-36) would be NaN or
+    infinite otherwise.  Raises ValueError."""
+    h = s.detach().cpu().numpy() if isinstance(s, torch.Tensor) else np.asarray(s)
+    h = np.asarray(h, dtype=np.float64).reshape(-1)
+    if not np.all(np.isfinite(h) & (h > 0.0)):
+        raise ValueError("aperture dispersions 's' must be positive and finite")
+    return dev(h, device=device)
 
 
 def dev(a, dtype=torch.float64, device='cuda'):

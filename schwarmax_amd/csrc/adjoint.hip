@@ -173,11 +173,13 @@ __device__ __forceinline__ double bin_vjp(const smx_grids& g, const int32_t* __r
 // contribution, and dL/dT_r = -scale_r / T_r * sum_e gbar[c][e] c_e slab_r[e].
 // One wave per integration.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void tdot_kernel(const double* __restrict__ ws, int64_t n_int,
+__global__ __launch_bounds__(256) void tdot_kernel(const long long* __restrict__ ws, int64_t n_int,
                                                    int r, int P, int B,
                                                    const double* __restrict__ valid,
                                                    const double* __restrict__ T_int,
                                                    const double* __restrict__ npb, double area,
+                                                   const double* __restrict__ dt0, smx_schedule s,
+                                                   int NC, const double* __restrict__ sd,
                                                    const double* __restrict__ gbar,
                                                    double* __restrict__ scale,
                                                    double* __restrict__ tbar) {
@@ -190,20 +192,24 @@ __global__ __launch_bounds__(256) void tdot_kernel(const double* __restrict__ ws
   const double sc = valid[i] > 0.5 ? (1.0 / (nv + 0.1)) / T_int[i] : 0.0;
   const int64_t rows = (int64_t)P + 6 * (int64_t)B;
   const double* gr = gbar + c * rows;
-  const double* rz = ws + i * slab_stride(P, B);
-  const double* ap = rz + slab_rz_len(P);
+  const long long* rz = ws + i * slab_stride(P, B);
+  const long long* ap = rz + slab_rz_len(P);
   double dot = 0.0;
   if (sc != 0.0) {
+    // the slab's fixed-point sums in the forward's units (fix_exp_time)
+    const int e_t = fix_exp_time(dt0[i], s, NC);
+    const double un_t = fix_unscale(e_t);
     for (int64_t e = lane; e < rows; e += 64) {
       double v;
       if (e < P) {
-        v = rz[e];
+        v = static_cast<double>(rz[e]) * un_t;
       } else {
         const int64_t q = e - P;
         const int64_t blk = q / B, a = q - blk * B;
         const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
-        v = ap[kApRec * a + comp];
-        if (blk == 0) v = v / (npb[a] * area + kEps);
+        v = static_cast<double>(ap[kApRec * a + comp]) *
+            (comp == 0 ? un_t : fix_unscale(e_t + fix_exp_gh(sd[a])));
+        if (blk == 0) v = v / aperture_area(npb[a], area);
       }
       dot += gr[e] * v;
     }
@@ -813,8 +819,9 @@ int smx_library_vjp(const smx_potential* p, const smx_grids* g, const smx_schedu
   double* tbar = scale + n_int;
   stage_begin(ST_ADJOINT, st);
   hipLaunchKernelGGL(tdot_kernel, dim3(blocks_for(n_int, 4)), dim3(256), 0, st,
-                     static_cast<const double*>(fwd_workspace), n_int, (int)r, P, g->num_vbin, valid,
-                     T_int, num_per_bin, g->area_pixel, lib_bar, scale, tbar);
+                     static_cast<const long long*>(fwd_workspace), n_int, (int)r, P, g->num_vbin,
+                     valid, T_int, num_per_bin, g->area_pixel, dt0, *s, g->symmetry ? 4 : 1, sdisp,
+                     lib_bar, scale, tbar);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(library_vjp_kernel, dim3((unsigned)vjp_waves(n_int)), dim3(64), 0, st, *p, *g, *s,
                      wb, dt0, n_int, (int)r, bin_mapping, num_per_bin, v0, sdisp, lib_bar, scale,

@@ -9,7 +9,9 @@
 // of the persistent grid pops items from an atomic counter, integrates the
 // item's r x columns integrations into a slab set it owns (so the slab pool
 // is bounded by the grid size, not the batch), then writes the item's
-// realization-averaged library rows and re-zeroes the slabs it touched.
+// realization-averaged library rows and re-zeroes the slabs it touched.  The
+// slabs hold the same fixed-point sums as the single-model path's, so each
+// model's library equals its single-model library bit for bit.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <vector>
@@ -71,10 +73,11 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
     const smx_schedule* __restrict__ scheds, const BatchModel* __restrict__ models,
     const BatchItem* __restrict__ items, int n_items, int r, const int32_t* __restrict__ bm,
     const double* __restrict__ npb, const double* __restrict__ v0, const double* __restrict__ sd,
-    double* __restrict__ pool, unsigned* __restrict__ counter) {
+    long long* __restrict__ pool, unsigned* __restrict__ counter) {
   __shared__ double s_valid[64];
   __shared__ double s_T[64];
-  __shared__ double ap_lds[24 * 64];   // aperture-record cache, 4 copies x 6 terms
+  __shared__ int s_et[64];
+  __shared__ long long ap_lds[24 * 64];   // aperture-record cache, 4 copies x 6 terms
   __shared__ int s_item;
   const int lane = threadIdx.x;
   const smx_grids& g0 = grids[0];
@@ -82,8 +85,8 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
   const int B = g0.num_vbin;
   const int64_t stride = slab_stride(P, B);
   const int64_t rows = (int64_t)P + 6 * (int64_t)B;
-  double* rz = pool + ((int64_t)blockIdx.x * 64 + lane) * stride;
-  double* ap = rz + slab_rz_len(P);
+  long long* rz = pool + ((int64_t)blockIdx.x * 64 + lane) * stride;
+  long long* ap = rz + slab_rz_len(P);
   for (;;) {
     if (lane == 0) s_item = (int)atomicAdd(counter, 1u);
     __syncthreads();
@@ -97,7 +100,9 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
     const smx_schedule& s = scheds[w.model];
     const int cc = lane / r, k = lane - cc * r;
     const bool active = cc < w.ncols && cc * r + k < 64;
+    const int NC = g.symmetry ? 4 : 1;
     double valid = 0.0, T = 1.0;
+    int e_t = 0;
     if (active) {
       const int64_t q = (int64_t)(w.col0 + cc) * r + k;
       double y[6], k1[6];
@@ -106,17 +111,19 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
       deriv(p, s.Omega, y, k1);
       const double EJ0 = jacobi(p, s.Omega, y);
       double t = 0.0, dt = M.dt0[q];
+      e_t = fix_exp_time(M.dt0[q], s, NC);
       const double T_acc =
-          g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1,
-                                            ap_lds + lane)
-                     : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, rz, ap, nullptr, 0, 1,
-                                            ap_lds + lane);
+          g.symmetry ? integrate_and_bin<4>(p, g, s, bm, v0, sd, t, y, k1, dt, e_t, rz, ap, nullptr,
+                                            0, 1, ap_lds + lane)
+                     : integrate_and_bin<1>(p, g, s, bm, v0, sd, t, y, k1, dt, e_t, rz, ap, nullptr,
+                                            0, 1, ap_lds + lane);
       const double EJ = jacobi(p, s.Omega, y);
       valid = ((fabs(EJ / EJ0 - 1.0) < 0.1) && (t > M.T_total[q] / 10.0)) ? 1.0 : 0.0;
       T = T_acc + kEps;
     }
     s_valid[lane] = valid;
     s_T[lane] = T;
+    s_et[lane] = e_t;
     __syncthreads();
     // realization average of the item's columns (integrates.py:212-236,
     // 287-300), rows [Rzphi | xy | h1 | h2 | h3 | h4 | h0]; the slabs read are
@@ -129,21 +136,23 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
       for (int64_t e = lane; e < rows; e += 64) {
         double acc = 0.0;
         for (int kk = 0; kk < r; ++kk) {
-          double* srz = pool + ((int64_t)blockIdx.x * 64 + c * r + kk) * stride;
-          double* sap = srz + slab_rz_len(P);
-          double v;
+          long long* srz = pool + ((int64_t)blockIdx.x * 64 + c * r + kk) * stride;
+          long long* sap = srz + slab_rz_len(P);
+          const int et = s_et[c * r + kk];
+          const bool ok = s_valid[c * r + kk] > 0.5;
           if (e < P) {
-            v = srz[e] / s_T[c * r + kk];
-            srz[e] = 0.0;
+            lib_add(acc, static_cast<double>(srz[e]) * fix_unscale(et), s_T[c * r + kk], false, 1.0,
+                    ok, wt);
+            srz[e] = 0;
           } else {
             const int64_t q = e - P;
             const int64_t blk = q / B, a = q - blk * B;
-            const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : (int)blk + 1);
-            v = sap[kApRec * a + comp] / s_T[c * r + kk];
-            sap[kApRec * a + comp] = 0.0;
-            if (blk == 0) v = v / (npb[a] * g.area_pixel + kEps);
+            const int comp = block_comp((int)blk);
+            const double un = fix_unscale(comp == 0 ? et : et + fix_exp_gh(sd[a]));
+            lib_add(acc, static_cast<double>(sap[kApRec * a + comp]) * un, s_T[c * r + kk], blk == 0,
+                    aperture_area(npb[a], g.area_pixel), ok, wt);
+            sap[kApRec * a + comp] = 0;
           }
-          acc += (s_valid[c * r + kk] > 0.5 ? v : 0.0) * wt;
         }
         out[e] = acc;
       }
@@ -152,10 +161,10 @@ __global__ __launch_bounds__(64) void integrate_batch_kernel(
   }
 }
 
-__global__ void zero_kernel(double* __restrict__ p, int64_t n) {
+__global__ void zero_kernel(long long* __restrict__ p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0;
+    p[i] = 0;
 }
 
 constexpr int BATCH_WAVES = 1024;   // persistent waves (4 per CU on 256 CUs)
@@ -225,7 +234,7 @@ int smx_build_library_batch(int32_t n_models, const smx_potential* pots, const s
   hipStream_t st = as_stream(stream);
   char* base = static_cast<char*>(workspace);
   const int64_t pool_words = slab_stride(P, B) * 64 * BATCH_WAVES;
-  double* pool = reinterpret_cast<double*>(base);
+  long long* pool = reinterpret_cast<long long*>(base);
   size_t off = align256(pool_words * sizeof(double));
   unsigned* counter = reinterpret_cast<unsigned*>(base + off);
   off += 256;
@@ -259,7 +268,7 @@ int smx_build_library_batch(int32_t n_models, const smx_potential* pots, const s
   stage_end(ST_ICS, st, 1);
   stage_begin(ST_INTEGRATE, st);
   hipLaunchKernelGGL(zero_kernel, dim3(4096), dim3(256), 0, st, pool, pool_words);
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<double*>(counter), 1);
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<long long*>(counter), 1);
   SMX_LAUNCHED();
   hipLaunchKernelGGL(integrate_batch_kernel, dim3(BATCH_WAVES), dim3(64), 0, st, d_pots, d_grids,
                      d_scheds, d_models, d_items, (int)hi.size(), (int)r, bin_mapping, num_per_bin,

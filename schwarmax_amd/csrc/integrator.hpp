@@ -117,89 +117,136 @@ __device__ __forceinline__ double bs3_attempt_stages(const smx_potential& p, con
 }
 
 // per-integration accumulator slab: P cells (padded to 8) then B records of
-// kApRec = 6 doubles [dt, g dt H0, ..., g dt H4] (48 bytes, 16-byte aligned)
+// kApRec = 6 terms [dt, g dt H0, ..., g dt H4] (48 bytes, 16-byte aligned),
+// every entry a 64-bit fixed-point sum (FixScale)
 constexpr int kApRec = 6;
 __host__ __device__ __forceinline__ int64_t slab_rz_len(int P) { return (P + 7) & ~7; }
 __host__ __device__ __forceinline__ int64_t slab_stride(int P, int B) {
   return slab_rz_len(P) + kApRec * (int64_t)B;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed-point sums.  Each binned contribution (a step length, or a step
+// length times a Gauss-Hermite term) is rounded to a multiple of a power of
+// two and the multiples are added as 64-bit integers.  Integer addition is
+// exact and associative, so a cell's or record term's sum does not depend on
+// the order or grouping of its contributions: the fused integrator (register
+// caches, one lane per integration), the persistent batched kernel and the
+// LDS histograms of the sample-binning kernel all produce the same bits.
+// The power of two comes from a bound on the sum, so it never overflows:
+//   * step lengths: every accepted step is at most max(dt0, dt_max) (the first
+//     attempt uses dt0, later ones are clamped to dt_max), so an integration's
+//     NC copies sum to less than B_t = NC N_max max(dt0, dt_max) < 2^e_t;
+//   * g H_k terms: g = exp(-y^2/2) / (sqrt(2 pi) s) and |exp(-y^2/2) H_k(y)|
+//     <= 1 for k <= 4, so |g H_k| < 2 / (sqrt(2 pi) s) < 2^e_g(s).
+// Sums are held at 2^-(61 - e) resolution (< 2^61 in magnitude): about 2^-60
+// of the bound, 1e-13 of a typical step's contribution at the 20,000-orbit
+// shapes, far below the FP64 library's 1e-8 parity bar.
+// ---------------------------------------------------------------------------
+constexpr int kFixBits = 61;
+
+__host__ __device__ __forceinline__ int fix_exp_of(double bound) {
+  int e = 0;
+  frexp(bound, &e);                       // bound < 2^e for a finite bound > 0
+  if (!(bound > 0.0) || !(bound < 1e300)) e = 0;   // non-finite: its integration is invalid
+  return e < -900 ? -900 : (e > 900 ? 900 : e);
+}
+
+// e_t of one integration
+__host__ __device__ __forceinline__ int fix_exp_time(double dt0, const smx_schedule& s, int NC) {
+  const double dtm = dt0 > s.dt_max ? dt0 : s.dt_max;
+  return fix_exp_of(static_cast<double>(NC) * static_cast<double>(s.N_max > 0 ? s.N_max : 1) * dtm);
+}
+
+// e_g of an aperture with dispersion s (> 0, finite: the host checks)
+__device__ __forceinline__ int fix_exp_gh(double s) { return fix_exp_of(2.0 * kInvSqrt2Pi / s); }
+
+// scale 2^(61 - e) and its inverse
+__device__ __forceinline__ double fix_scale(int e) { return ldexp(1.0, kFixBits - e); }
+__device__ __forceinline__ double fix_unscale(int e) { return ldexp(1.0, e - kFixBits); }
+
+__device__ __forceinline__ long long fix_q(double x, double scale) {
+  return static_cast<long long>(rint(x * scale));
+}
+
+// Area of an aperture of npb pixels (integrates.py:212-236), one rounding
+// per operation wherever the library is finalized.
+__host__ __device__ __forceinline__ double aperture_area(double npb, double area_pixel) {
+#pragma clang fp contract(off)
+  return npb * area_pixel + kEps;
+}
+
 // Register cache of the (R, z, phi) cell each symmetry copy last hit.  An
 // orbit's consecutive accepted steps mostly stay in one cell, so a copy's
-// weights are summed in a register and added to the slab when the copy moves
-// to another cell, and at the end of the integration: the slab receives the
-// same sums, with the additions grouped per visit.  Most of the binning's
-// cost is slab cache-line traffic (12.9 GB of slabs at 80,000 integrations),
-// which this removes for the cells.  Out-of-grid copies (id -1) are never
+// fixed-point weights are summed in a register and added to the slab when the
+// copy moves to another cell, and at the end of the integration: the slab
+// receives the same integer sum.  Out-of-grid copies (id -1) are never
 // written.
 template <int NC>
 struct CellCache {
   int cell[NC];
-  double w[NC];
+  long long w[NC];
   __device__ __forceinline__ void init() {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       cell[c] = -1;
-      w[c] = 0.0;
+      w[c] = 0;
     }
   }
-  __device__ __forceinline__ void add(double* __restrict__ rz, int c, int id, double wt) {
+  __device__ __forceinline__ void add(long long* __restrict__ rz, int c, int id, long long q) {
     if (id == cell[c]) {
-      w[c] += wt;
+      w[c] += q;
       return;
     }
     if (cell[c] >= 0) rz[cell[c]] += w[c];
     cell[c] = id;
-    w[c] = wt;
+    w[c] = q;
   }
-  __device__ __forceinline__ void flush(double* __restrict__ rz) {
+  __device__ __forceinline__ void flush(long long* __restrict__ rz) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (cell[c] >= 0) rz[cell[c]] += w[c];
       cell[c] = -1;
-      w[c] = 0.0;
+      w[c] = 0;
     }
   }
 };
 
 // Cache of the aperture record each symmetry copy last hit, the same scheme
-// as CellCache for the six record terms [dt, g dt H0..H4]: a copy's terms
-// are summed while its samples fall in one aperture and added to the slab's
-// record when it moves to another aperture, and at the end of the
-// integration.  A copy in no aperture (id -1) leaves its cached record as it
-// is.  The ids live in registers, the sums in LDS (the integrator's registers
-// are full): lane l's term k of copy c at sum[(6 c + k) * 64 + l], so a
-// wave's accesses are consecutive doubles.
+// as CellCache for the six record terms: a copy's terms are summed while its
+// samples fall in one aperture and added to the slab's record when it moves
+// to another aperture, and at the end of the integration.  A copy in no
+// aperture (id -1) leaves its cached record as it is.  The ids live in
+// registers, the sums in LDS (the integrator's registers are full): lane l's
+// term k of copy c at sum[(6 c + k) * 64 + l], so a wave's accesses are
+// consecutive.
 template <int NC>
 struct ApCache {
   int id[NC];
-  double* sum;   // this lane's column of a __shared__ (6 NC) x 64 array
-  __device__ __forceinline__ void init(double* lds_col) {
+  long long* sum;   // this lane's column of a __shared__ (6 NC) x 64 array
+  __device__ __forceinline__ void init(long long* lds_col) {
     sum = lds_col;
 #pragma unroll
     for (int c = 0; c < NC; ++c) id[c] = -1;
   }
-  __device__ __forceinline__ void write(double* __restrict__ ap, int c) const {
+  __device__ __forceinline__ void write(long long* __restrict__ ap, int c) const {
     if (id[c] < 0) return;
-    double* rec = ap + kApRec * (int64_t)id[c];
+    long long* rec = ap + kApRec * (int64_t)id[c];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) rec[k] += sum[(6 * c + k) * 64];
+    for (int k = 0; k < kApRec; ++k) rec[k] += sum[(6 * c + k) * 64];
   }
-  __device__ __forceinline__ void add(double* __restrict__ ap, int c, int a, double w, double gd,
-                                      const double (&H)[5]) {
+  __device__ __forceinline__ void add(long long* __restrict__ ap, int c, int a, const long long (&q)[kApRec]) {
     if (a != id[c]) {
       write(ap, c);
       id[c] = a;
-      sum[(6 * c) * 64] = w;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) sum[(6 * c + 1 + k) * 64] = gd * H[k];
+      for (int k = 0; k < kApRec; ++k) sum[(6 * c + k) * 64] = q[k];
       return;
     }
-    sum[(6 * c) * 64] += w;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) sum[(6 * c + 1 + k) * 64] += gd * H[k];
+    for (int k = 0; k < kApRec; ++k) sum[(6 * c + k) * 64] += q[k];
   }
-  __device__ __forceinline__ void flush(double* __restrict__ ap) {
+  __device__ __forceinline__ void flush(long long* __restrict__ ap) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       write(ap, c);
@@ -208,27 +255,73 @@ struct ApCache {
   }
 };
 
+// One realization's term of a library entry: the fixed-point sum S (or the
+// FP64 sum of the trajectory path) over T, the count term also over the
+// aperture's area, added into the realization average with weight wt when
+// the integration is valid.  Shared by the library finalizers (fused, batched
+// and sample-binning kernels), so all round alike.
+__device__ __forceinline__ void lib_add(double& out, double sum, double T, bool count, double area,
+                                        bool ok, double wt) {
+#pragma clang fp contract(off)
+  double v = sum / T;
+  if (count) v = v / area;
+  out += (ok ? v : 0.0) * wt;
+}
+
+// Library row of slab entry (aperture a, record term comp): rows
+// [Rzphi (P) | counts | h1 | h2 | h3 | h4 | h0] (B each); record terms
+// [dt, g dt H0, g dt H1..H4].
+__host__ __device__ __forceinline__ int rec_block(int comp) {
+  return comp == 0 ? 0 : (comp == 1 ? 5 : comp - 1);
+}
+__host__ __device__ __forceinline__ int block_comp(int blk) {
+  return blk == 0 ? 0 : (blk == 5 ? 1 : blk + 1);
+}
+
+// The grid cell and aperture of copy c of a projected sample, with the
+// fixed-point record terms when the copy falls in an aperture (a >= 0).
+// Shared by the fused integrator and the sample-binning kernel, so both add
+// the same integers.  w: the step length; e_t: the integration's exponent
+// (fix_exp_time).
+__device__ __forceinline__ void copy_bins(const smx_grids& g, const int32_t* __restrict__ bm,
+                                          const double* __restrict__ v0,
+                                          const double* __restrict__ sd, const SampleProj& sp,
+                                          double z, int c, double w, int e_t, int& cell, int& a,
+                                          long long (&q)[kApRec]) {
+  double zs, phi, xr, yr, vlos;
+  copy_coords(sp, c, zs, phi, xr, yr, vlos);
+  cell = rzphi_cell(g, sp.R, zs * z, phi);
+  a = aperture(g, bm, xr, yr);
+  if (a >= 0) {
+    double gw, H[5];
+    gh_terms(vlos, v0[a], sd[a], gw, H);
+    const double gd = gw * w;
+    const double sg = fix_scale(e_t + fix_exp_gh(sd[a]));
+    q[0] = fix_q(w, fix_scale(e_t));
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[1 + k] = fix_q(gd * H[k], sg);
+  }
+}
+
 // Binning of one accepted sample's NC symmetry copies with weight w: cells
 // and aperture records through their register caches.
 template <int NC>
 __device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __restrict__ bm,
                                            const double* __restrict__ v0,
                                            const double* __restrict__ sd, const double* y,
-                                           double w, CellCache<NC>& cache, ApCache<NC>& acache,
-                                           double* __restrict__ rz, double* __restrict__ ap) {
+                                           double w, int e_t, CellCache<NC>& cache,
+                                           ApCache<NC>& acache, long long* __restrict__ rz,
+                                           long long* __restrict__ ap) {
   SampleProj sp;
   project(g, y, NC > 2, sp);
+  const long long qd = fix_q(w, fix_scale(e_t));
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    double zs, phi, xr, yr, vlos;
-    copy_coords(sp, c, zs, phi, xr, yr, vlos);
-    cache.add(rz, c, rzphi_cell(g, sp.R, zs * y[2], phi), w);
-    const int a = aperture(g, bm, xr, yr);
-    if (a >= 0) {
-      double gw, H[5];
-      gh_terms(vlos, v0[a], sd[a], gw, H);
-      acache.add(ap, c, a, w, gw * w, H);
-    }
+    int cell, a;
+    long long q[kApRec];
+    copy_bins(g, bm, v0, sd, sp, y[2], c, w, e_t, cell, a, q);
+    cache.add(rz, c, cell, qd);
+    if (a >= 0) acache.add(ap, c, a, q);
   }
 }
 
@@ -236,18 +329,19 @@ __device__ __forceinline__ void bin_sample(const smx_grids& g, const int32_t* __
 // (integrates.py:181-205), every accepted state binned with weight dt in its
 // NC copies; returns the accumulated time (NC x the integrated time).  With
 // ckpt, the state (y, dt, t) before every ckpt_every-th attempt is stored at
-// ckpt + (attempt / ckpt_every) * ckpt_stride (8 doubles).  ap_lds: this
-// lane's column of a __shared__ array of at least 6 NC x 64 doubles (the
+// ckpt + (attempt / ckpt_every) * ckpt_stride (8 doubles).  e_t: the
+// integration's fixed-point exponent (fix_exp_time).  ap_lds: this lane's
+// column of a __shared__ array of at least 6 NC x 64 integers (the
 // aperture-record cache; the block is one wave of 64 lanes).
 template <int NC>
 __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, const smx_grids& g,
                                                     const smx_schedule& s, const int32_t* bm,
                                                     const double* v0, const double* sd, double& t,
-                                                    double* y, double* k1, double& dt,
-                                                    double* __restrict__ rz,
-                                                    double* __restrict__ ap, double* ckpt,
+                                                    double* y, double* k1, double& dt, int e_t,
+                                                    long long* __restrict__ rz,
+                                                    long long* __restrict__ ap, double* ckpt,
                                                     int64_t ckpt_stride, int ckpt_every,
-                                                    double* ap_lds) {
+                                                    long long* ap_lds) {
   CellCache<NC> cache;
   cache.init();
   ApCache<NC> acache;
@@ -263,7 +357,7 @@ __device__ __forceinline__ double integrate_and_bin(const smx_potential& p, cons
     }
     const double used = bs3_attempt(p, s, t, y, k1, dt);
     if (used != 0.0) {
-      bin_sample<NC>(g, bm, v0, sd, y, used, cache, acache, rz, ap);
+      bin_sample<NC>(g, bm, v0, sd, y, used, e_t, cache, acache, rz, ap);
       T_acc += NC * used;
     }
   }

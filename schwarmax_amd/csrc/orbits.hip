@@ -13,6 +13,7 @@
 #include "potential.hpp"
 #include "binning.hpp"
 #include "integrator.hpp"
+#include "tuning.hpp"
 #include "ics.hpp"
 #include "timing.hpp"
 
@@ -107,9 +108,11 @@ __global__ __launch_bounds__(256, SMX_IC_WAVES) void initial_conditions_kernel(
 // one wave, which keeps its LDS operations in order).  Each lane runs, in
 // loop steps: its head [0, hb), its whole integrations from hb on, its tail
 // [C - (L - ta), C).  Every integration's attempts, binning and checkpoints
-// are those of integrate_and_bin, in the same order, into its own slab; the
-// register caches are flushed at a head / tail boundary, which groups the
-// slab additions differently (FP64 reassociation only).
+// are those of integrate_and_bin, in the same order, into its own slab (its
+// fixed-point sums do not depend on how the register caches group them at a
+// head / tail boundary).  In sample mode the integrator does not bin: every
+// attempt stores its state and step length (0 when rejected) in the
+// integration's sample rows, and bin_library_kernel bins them.
 // ---------------------------------------------------------------------------
 #ifndef SMX_INTEGRATOR_WAVES
 #define SMX_INTEGRATOR_WAVES 1   // waves per SIMD the integrator's registers are sized for
@@ -120,17 +123,28 @@ struct IntegState {
   double t, dt, T_acc, EJ0;
   int64_t job;       // integration index, or -1
   int64_t attempt;   // attempts made so far
+  int e_t;           // fixed-point exponent of the integration (fix_exp_time)
 };
+
+// Sample rows: integration i's attempt a at samples + (i N_max + a) * kSampleW,
+// [y (6), step length (0 when rejected), unused]; one 64-byte line per attempt.
+constexpr int kSampleW = 8;
+
+// Bytes of the sample rows of n_int integrations (sample mode).
+static int64_t sample_bytes(const smx_schedule* s, int64_t n_int) {
+  return n_int * (int64_t)(s->N_max > 0 ? s->N_max : 0) * kSampleW * (int64_t)sizeof(double);
+}
 constexpr int kHandoff = 16;   // doubles per handed-over state
 
-template <int NC>
+template <int NC, bool SAMPLES>
 __device__ __forceinline__ void integrate_balanced(
     const smx_potential& p, const smx_grids& g, const smx_schedule& s,
     const double* __restrict__ wb, const double* __restrict__ T_total,
     const double* __restrict__ dt0, int64_t n_int, int64_t W, const int32_t* __restrict__ bm,
-    const double* __restrict__ v0, const double* __restrict__ sd, double* __restrict__ ws,
-    double* __restrict__ valid_out, double* __restrict__ T_out, double* __restrict__ tfin_out,
-    double* __restrict__ ckpt, int ckpt_every, double* ap_col, double* handoff) {
+    const double* __restrict__ v0, const double* __restrict__ sd, long long* __restrict__ ws,
+    double* __restrict__ samples, double* __restrict__ valid_out, double* __restrict__ T_out,
+    double* __restrict__ tfin_out, double* __restrict__ ckpt, int ckpt_every, long long* ap_col,
+    double* handoff) {
   const int lane = threadIdx.x;
   const int64_t w = blockIdx.x;
   const int64_t j0 = w * n_int / W, m = (w + 1) * n_int / W - j0;
@@ -159,8 +173,9 @@ __device__ __forceinline__ void integrate_balanced(
   ApCache<NC> acache;
   cache.init();
   acache.init(ap_col);
-  double* rz = nullptr;
-  double* ap = nullptr;
+  long long* rz = nullptr;
+  long long* ap = nullptr;
+  st.e_t = 0;
   int sidx = 0, kind = 0;
   int64_t job = 0, first = 0, t0 = 0, t1 = 0;
   if (nseg > 0) ls.segment(0, kind, job, first, t0, t1);
@@ -174,8 +189,10 @@ __device__ __forceinline__ void integrate_balanced(
     if (__ballot(ends || starts_now) != 0) {   // wave-uniform
       bool start = starts_now;
       if (ends) {
-        cache.flush(rz);
-        acache.flush(ap);
+        if (!SAMPLES) {
+          cache.flush(rz);
+          acache.flush(ap);
+        }
         const int64_t i = j0 + st.job;
         if (kind == 0) {
           double* h = handoff + lane;
@@ -228,8 +245,11 @@ __device__ __forceinline__ void integrate_balanced(
         }
         st.job = job;
         st.attempt = first;
-        rz = ws + i * stride;
-        ap = rz + slab_rz_len(P);
+        st.e_t = fix_exp_time(dt0[i], s, NC);
+        if (!SAMPLES) {
+          rz = ws + i * stride;
+          ap = rz + slab_rz_len(P);
+        }
       }
     }
     if (st.job >= 0 && step < ls.C) {
@@ -242,8 +262,15 @@ __device__ __forceinline__ void integrate_balanced(
         c[7] = st.t;
       }
       const double used = bs3_attempt(p, s, st.t, st.y, st.k1, st.dt);
-      if (used != 0.0) {
-        bin_sample<NC>(g, bm, v0, sd, st.y, used, cache, acache, rz, ap);
+      if (SAMPLES) {
+        double* row = samples + (i * L + st.attempt) * kSampleW;
+        *reinterpret_cast<double2*>(row) = make_double2(st.y[0], st.y[1]);
+        *reinterpret_cast<double2*>(row + 2) = make_double2(st.y[2], st.y[3]);
+        *reinterpret_cast<double2*>(row + 4) = make_double2(st.y[4], st.y[5]);
+        *reinterpret_cast<double2*>(row + 6) = make_double2(used, 0.0);
+        if (used != 0.0) st.T_acc += NC * used;
+      } else if (used != 0.0) {
+        bin_sample<NC>(g, bm, v0, sd, st.y, used, st.e_t, cache, acache, rz, ap);
         st.T_acc += NC * used;
       }
       ++st.attempt;
@@ -251,25 +278,32 @@ __device__ __forceinline__ void integrate_balanced(
   }
 }
 
+// SAMPLES: store every attempt's state in `samples` instead of binning into
+// the slabs `ws` (one of the two is null).
+template <bool SAMPLES>
 __global__ __launch_bounds__(64, SMX_INTEGRATOR_WAVES) void integrate_library_kernel(
     smx_potential p, smx_grids g, smx_schedule s, const double* __restrict__ wb,
     const double* __restrict__ T_total, const double* __restrict__ dt0, int64_t n_int,
     const int32_t* __restrict__ bm, const double* __restrict__ v0, const double* __restrict__ sd,
-    double* __restrict__ ws, double* __restrict__ valid_out, double* __restrict__ T_out,
-    double* __restrict__ tfin_out, double* __restrict__ ckpt, int ckpt_every) {
-  __shared__ double ap_lds[24 * 64];         // aperture-record cache, 4 copies x 6 terms
-  __shared__ double handoff[kHandoff * 64];  // head -> tail states, one per lane
+    long long* __restrict__ ws, double* __restrict__ samples, double* __restrict__ valid_out,
+    double* __restrict__ T_out, double* __restrict__ tfin_out, double* __restrict__ ckpt,
+    int ckpt_every) {
+  __shared__ long long ap_lds[SAMPLES ? 1 : 24 * 64];   // aperture-record cache, 4 copies x 6 terms
+  __shared__ double handoff[kHandoff * 64];            // head -> tail states, one per lane
   const int64_t W = gridDim.x;
   if (g.symmetry)
-    integrate_balanced<4>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, valid_out, T_out,
-                          tfin_out, ckpt, ckpt_every, ap_lds + threadIdx.x, handoff);
+    integrate_balanced<4, SAMPLES>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, samples,
+                                   valid_out, T_out, tfin_out, ckpt, ckpt_every,
+                                   ap_lds + (SAMPLES ? 0 : threadIdx.x), handoff);
   else
-    integrate_balanced<1>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, valid_out, T_out,
-                          tfin_out, ckpt, ckpt_every, ap_lds + threadIdx.x, handoff);
+    integrate_balanced<1, SAMPLES>(p, g, s, wb, T_total, dt0, n_int, W, bm, v0, sd, ws, samples,
+                                   valid_out, T_out, tfin_out, ckpt, ckpt_every,
+                                   ap_lds + (SAMPLES ? 0 : threadIdx.x), handoff);
 }
 
 // Waves of the balanced integrator: the waves resident at once when there
 // are more integrations than their lanes, else one lane per integration.
+template <bool SAMPLES>
 static int64_t integrator_waves(int64_t n_int) {
   static int resident = 0;
   static std::mutex mu;
@@ -279,8 +313,8 @@ static int64_t integrator_waves(int64_t n_int) {
       int dev = 0, cus = 0, per_cu = 0;
       if (hipGetDevice(&dev) == hipSuccess &&
           hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, integrate_library_kernel, 64, 0) ==
-              hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, integrate_library_kernel<SAMPLES>,
+                                                       64, 0) == hipSuccess &&
           cus > 0 && per_cu > 0)
         resident = cus * per_cu;
       else
@@ -298,12 +332,18 @@ static int64_t integrator_waves(int64_t n_int) {
 // item reads its 48-byte record once per realization and writes the six
 // aperture rows (counts, h1..h4, h0), rather than six items each reading one
 // value of the record (whose other values then come back from beyond L2 for
-// the other five).
-__global__ void finalize_library_kernel(const double* __restrict__ ws, int64_t n, int r, int P,
+// the other five).  FIXED: fixed-point slabs of the library paths (scales
+// from dt0, the schedule and the apertures' dispersions); otherwise FP64
+// slabs of the trajectory path, already divided by T.
+template <bool FIXED>
+__global__ void finalize_library_kernel(const void* __restrict__ ws_, int64_t n, int r, int P,
                                         int B, const double* __restrict__ valid,
                                         const double* __restrict__ T_int,
                                         const double* __restrict__ num_per_bin, double area_pixel,
-                                        int pre_normalised, double* __restrict__ lib) {
+                                        const double* __restrict__ dt0, smx_schedule s, int NC,
+                                        const double* __restrict__ sd, double* __restrict__ lib) {
+  const long long* wsq = static_cast<const long long*>(ws_);
+  const double* wsd = static_cast<const double*>(ws_);
   const int64_t rows = (int64_t)P + 6 * (int64_t)B;
   const int64_t items = (int64_t)P + (int64_t)B;
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -318,33 +358,151 @@ __global__ void finalize_library_kernel(const double* __restrict__ ws, int64_t n
     double out = 0.0;
     for (int k = 0; k < r; ++k) {
       const int64_t it = c * r + k;
-      const double T = pre_normalised ? 1.0 : T_int[it];
-      const double v = ws[it * stride + j] / T;
-      out += (valid[it] > 0.5 ? v : 0.0) * wt;
+      const int64_t e = it * stride + j;
+      const double sum = FIXED ? static_cast<double>(wsq[e]) * fix_unscale(fix_exp_time(dt0[it], s, NC))
+                               : wsd[e];
+      lib_add(out, sum, FIXED ? T_int[it] : 1.0, false, 1.0, valid[it] > 0.5, wt);
     }
     lib[c * rows + j] = out;
     return;
   }
   const int64_t a = j - P;
-  const double area = num_per_bin[a] * area_pixel + kEps;
-  // output blocks: 0 counts (surface density), 1..4 -> h1..h4, 5 -> h0;
-  // record components: 0 counts, 1 h0, 2..5 h1..h4
+  const double area = aperture_area(num_per_bin[a], area_pixel);
+  const int e_g = FIXED ? fix_exp_gh(sd[a]) : 0;
   double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < r; ++k) {
     const int64_t it = c * r + k;
-    const double* rec = ws + it * stride + slab_rz_len(P) + kApRec * a;
-    const double T = pre_normalised ? 1.0 : T_int[it];
+    const int64_t e = it * stride + slab_rz_len(P) + kApRec * a;
+    const int e_t = FIXED ? fix_exp_time(dt0[it], s, NC) : 0;
     const bool ok = valid[it] > 0.5;
 #pragma unroll
     for (int blk = 0; blk < 6; ++blk) {
-      const int comp = (blk == 0) ? 0 : (blk == 5 ? 1 : blk + 1);
-      double v = rec[comp] / T;
-      if (blk == 0) v = v / area;
-      out[blk] += (ok ? v : 0.0) * wt;
+      const int comp = block_comp(blk);
+      const double sum = FIXED ? static_cast<double>(wsq[e + comp]) *
+                                     fix_unscale(comp == 0 ? e_t : e_t + e_g)
+                               : wsd[e + comp];
+      lib_add(out[blk], sum, FIXED ? T_int[it] : 1.0, blk == 0, area, ok, wt);
     }
   }
 #pragma unroll
   for (int blk = 0; blk < 6; ++blk) lib[c * rows + P + blk * (int64_t)B + a] = out[blk];
+}
+
+// ---------------------------------------------------------------------------
+// Binning of stored samples into LDS-privatised histograms, one workgroup per
+// orbit column (integrates.py:115-174, 207-238, 287-300).  The workgroup's
+// LDS holds a window of the column's library rows as fixed-point sums (the
+// whole row set, 146 KB, at 12,288 cells and 1,000 apertures; larger grids
+// take several windows, each pass re-binning the samples).  For each valid
+// realization in order: the window is cleared, every thread bins its share
+// of the integration's accepted samples (projection, four symmetry copies,
+// cell and aperture ids, Gauss-Hermite terms) and adds their fixed-point
+// contributions with 64-bit LDS atomics, then each thread converts the
+// entries it owns and adds them into its register copy of the realization
+// average, in realization order as finalize_library_kernel does; the rows
+// are written once at the end.  The integer sums equal the fused
+// integrator's slab sums bit for bit, so the library does too.
+// Thread (wave w, lane l) takes attempts l L + j, j = w, w + waves, ...
+// (L = ceil(N / 64)): a wave's lanes are L attempts apart on the orbit, in
+// different cells mostly, so its atomics rarely collide.
+// ---------------------------------------------------------------------------
+constexpr int BL_THREADS = 512;
+constexpr int BL_PER_THREAD = 40;   // entries per thread: windows of up to 20,480 rows (160 KB)
+
+template <int NC>
+__global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
+    smx_grids g, smx_schedule s, const double* __restrict__ samples, int64_t n, int r,
+    const double* __restrict__ dt0, const double* __restrict__ valid,
+    const double* __restrict__ T_int, const int32_t* __restrict__ bm,
+    const double* __restrict__ v0, const double* __restrict__ sd,
+    const double* __restrict__ num_per_bin, int64_t win, double* __restrict__ lib) {
+  extern __shared__ long long hist[];
+  const int64_t c = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int P = g.nR * g.nz * g.nphi;
+  const int B = g.num_vbin;
+  const int64_t rows = (int64_t)P + 6 * (int64_t)B;
+  const int64_t N = s.N_max;
+  const int64_t Ls = (N + 63) / 64;
+  const int lane = tid & 63, wave = tid >> 6;
+  double nv = 0.0;
+  for (int k = 0; k < r; ++k) nv += valid[c * r + k];
+  const double wt = 1.0 / (nv + 0.1);
+  for (int64_t e0 = 0; e0 < rows; e0 += win) {
+    const int64_t e1 = rows < e0 + win ? rows : e0 + win;
+    double out[BL_PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < BL_PER_THREAD; ++q) out[q] = 0.0;
+    for (int k = 0; k < r; ++k) {
+      const int64_t it = c * r + k;
+      if (!(valid[it] > 0.5)) continue;   // uniform over the workgroup
+      for (int64_t e = e0 + tid; e < e1; e += BL_THREADS) hist[e - e0] = 0;
+      __syncthreads();
+      const int e_t = fix_exp_time(dt0[it], s, NC);
+      const double sc_t = fix_scale(e_t);
+      const double* base = samples + it * N * kSampleW;
+      for (int64_t j = wave; j < Ls; j += BL_THREADS / 64) {
+        const int64_t a_idx = lane * Ls + j;
+        if (a_idx >= N) continue;
+        const double* row = base + a_idx * kSampleW;
+        const double2 y01 = *reinterpret_cast<const double2*>(row);
+        const double2 y23 = *reinterpret_cast<const double2*>(row + 2);
+        const double2 y45 = *reinterpret_cast<const double2*>(row + 4);
+        const double used = row[6];
+        if (used == 0.0) continue;
+        const double y[6] = {y01.x, y01.y, y23.x, y23.y, y45.x, y45.y};
+        SampleProj sp;
+        project(g, y, NC > 2, sp);
+        const long long qd = fix_q(used, sc_t);
+#pragma unroll 1
+        for (int cp = 0; cp < NC; ++cp) {
+          int cell, a;
+          long long q[kApRec];
+          copy_bins(g, bm, v0, sd, sp, y[2], cp, used, e_t, cell, a, q);
+          if (cell >= e0 && cell < e1)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&hist[cell - e0]),
+                      static_cast<unsigned long long>(qd));
+          if (a >= 0) {
+#pragma unroll
+            for (int comp = 0; comp < kApRec; ++comp) {
+              const int64_t e = P + rec_block(comp) * (int64_t)B + a;
+              if (e >= e0 && e < e1)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&hist[e - e0]),
+                          static_cast<unsigned long long>(q[comp]));
+            }
+          }
+        }
+      }
+      __syncthreads();
+      const double T = T_int[it];
+      const double un_t = fix_unscale(e_t);
+#pragma unroll
+      for (int q = 0; q < BL_PER_THREAD; ++q) {
+        // the entry's offset, opaque to the compiler: otherwise it hoists the
+        // per-entry areas and exponents out of the realization loop and keeps
+        // all of them live through the binning
+        int off = tid + q * BL_THREADS;
+        asm volatile("" : "+v"(off));
+        const int64_t e = e0 + off;
+        if (e < e1) {
+          const double S = static_cast<double>(hist[e - e0]);
+          if (e < P) {
+            lib_add(out[q], S * un_t, T, false, 1.0, true, wt);
+          } else {
+            const int64_t blk = (e - P) / B, a = (e - P) - blk * B;
+            const double un = blk == 0 ? un_t : fix_unscale(e_t + fix_exp_gh(sd[a]));
+            lib_add(out[q], S * un, T, blk == 0, aperture_area(num_per_bin[a], g.area_pixel), true, wt);
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < BL_PER_THREAD; ++q) {
+      const int64_t e = e0 + tid + (int64_t)q * BL_THREADS;
+      if (e < e1) lib[c * rows + e] = out[q];
+    }
+  }
 }
 
 // Trajectory path (integrates.py:319-512): integrate storing every attempt's
@@ -609,43 +767,108 @@ int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int) {
   return slab_stride(g->nR * g->nz * g->nphi, g->num_vbin) * n_int * (int64_t)sizeof(double);
 }
 
+int64_t smx_library_build_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int) {
+  if (!g || !s || n_int < 0) return -1;
+  const int64_t slabs = smx_library_workspace_bytes(g, n_int);
+  const int64_t smp = sample_bytes(s, n_int);
+  return slabs > smp ? slabs : smp;
+}
+
 static int check_grids(const smx_grids* g) {
   SMX_REQUIRE(g->nR > 0 && g->nz > 0 && g->nphi > 0 && g->nX > 0 && g->nY > 0 && g->num_vbin > 0,
               "grid sizes must be positive");
   return 0;
 }
 
+// Rows per bin_library_kernel window: the whole row set when it fits the
+// workgroup's LDS, else as many as fit (at most BL_THREADS x BL_PER_THREAD).
+static int library_window(int64_t rows, int64_t* win) {
+  static int lds = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (lds == 0) {
+    int dev = 0, v = 0;
+    SMX_HIP(hipGetDevice(&dev));
+    SMX_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    SMX_REQUIRE(v >= 8 * BL_THREADS, "bin_library_kernel: %d bytes of LDS per workgroup", v);
+    lds = v;
+    const int64_t cap = (int64_t)BL_THREADS * BL_PER_THREAD * (int64_t)sizeof(long long);
+    const int bytes = (int)(lds < cap ? lds : cap);
+    SMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_library_kernel<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    SMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_library_kernel<4>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  }
+  int64_t cap = lds / (int64_t)sizeof(long long);
+  if (cap > (int64_t)BL_THREADS * BL_PER_THREAD) cap = (int64_t)BL_THREADS * BL_PER_THREAD;
+  *win = rows < cap ? rows : cap;
+  return 0;
+}
+
+// The library of n columns x r realizations.  samples: the sample mode
+// (integrate_library_kernel<true>, then bin_library_kernel), whose workspace
+// holds the sample rows; otherwise the fused integrator with fixed-point
+// slabs in the workspace, then finalize_library_kernel.  Both give the same
+// bits.  Checkpoints (the adjoint's, whose T-cotangent pre-pass reads the
+// slabs) use the fused form.
 static int build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                          const double* wb, const double* T_total, const double* dt0, int64_t n,
                          int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
                          const double* v0, const double* sdisp, void* workspace, double* lib,
                          double* valid, double* T_int, double* t_final, double* ckpt,
-                         int32_t ckpt_every, void* stream) {
+                         int32_t ckpt_every, bool samples, void* stream) {
   SMX_REQUIRE(p && g && s && wb && T_total && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
                   workspace && lib && valid && T_int && t_final,
               "smx_build_library: null pointer");
   SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0, "smx_build_library: bad sizes");
   SMX_REQUIRE(ckpt == nullptr || ckpt_every >= 1, "smx_build_library_ckpt: ckpt_every must be >= 1");
+  SMX_REQUIRE(!(samples && ckpt), "smx_build_library: checkpoints need the fused integrator");
   if (int e = check_grids(g)) return e;
   if (n == 0) return 0;
   const int64_t n_int = n * r;
   const int P = g->nR * g->nz * g->nphi;
-  const int64_t words = slab_stride(P, g->num_vbin) * n_int;
+  const int B = g->num_vbin;
+  const int NC = g->symmetry ? 4 : 1;
   hipStream_t st = as_stream(stream);
-  double* ws = static_cast<double*>(workspace);
-  hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, ws, words, 0.0);
+  if (samples) {
+    int64_t win = 0;
+    if (int e = library_window((int64_t)P + 6 * (int64_t)B, &win)) return e;
+    double* smp = static_cast<double*>(workspace);
+    stage_begin(ST_INTEGRATE, st);
+    hipLaunchKernelGGL(integrate_library_kernel<true>, dim3((unsigned)integrator_waves<true>(n_int)),
+                       dim3(64), 0, st, *p, *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp,
+                       nullptr, smp, valid, T_int, t_final, nullptr, 1);
+    SMX_LAUNCHED();
+    stage_end(ST_INTEGRATE, st, 1);
+    stage_begin(ST_FINALIZE, st);
+    if (NC == 4)
+      hipLaunchKernelGGL(bin_library_kernel<4>, dim3((unsigned)n), dim3(BL_THREADS),
+                         win * sizeof(long long), st, *g, *s, smp, n, (int)r, dt0, valid, T_int,
+                         bin_mapping, v0, sdisp, num_per_bin, win, lib);
+    else
+      hipLaunchKernelGGL(bin_library_kernel<1>, dim3((unsigned)n), dim3(BL_THREADS),
+                         win * sizeof(long long), st, *g, *s, smp, n, (int)r, dt0, valid, T_int,
+                         bin_mapping, v0, sdisp, num_per_bin, win, lib);
+    SMX_LAUNCHED();
+    stage_end(ST_FINALIZE, st, 1);
+    return 0;
+  }
+  const int64_t words = slab_stride(P, B) * n_int;
+  long long* ws = static_cast<long long*>(workspace);
+  hipLaunchKernelGGL(fill_kernel, dim3(4096), dim3(256), 0, st, reinterpret_cast<double*>(ws), words,
+                     0.0);
   SMX_LAUNCHED();
   stage_begin(ST_INTEGRATE, st);
-  hipLaunchKernelGGL(integrate_library_kernel, dim3((unsigned)integrator_waves(n_int)), dim3(64), 0,
-                     st, *p,
-                     *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp, ws, valid, T_int,
-                     t_final, ckpt, ckpt_every);
+  hipLaunchKernelGGL(integrate_library_kernel<false>, dim3((unsigned)integrator_waves<false>(n_int)),
+                     dim3(64), 0, st, *p, *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp,
+                     ws, nullptr, valid, T_int, t_final, ckpt, ckpt_every);
   SMX_LAUNCHED();
   stage_end(ST_INTEGRATE, st, 1);
-  const int64_t items = (int64_t)P + g->num_vbin;   // finalize work items per column
+  const int64_t items = (int64_t)P + B;   // finalize work items per column
   stage_begin(ST_FINALIZE, st);
-  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * items, 256)), dim3(256), 0, st, ws,
-                     n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 0, lib);
+  hipLaunchKernelGGL(finalize_library_kernel<true>, dim3(blocks_for(n * items, 256)), dim3(256), 0, st,
+                     ws, n, r, P, B, valid, T_int, num_per_bin, g->area_pixel, dt0, *s, NC, sdisp,
+                     lib);
   SMX_LAUNCHED();
   stage_end(ST_FINALIZE, st, 1);
   return 0;
@@ -657,7 +880,8 @@ int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_sche
                       const double* v0, const double* sdisp, void* workspace, double* lib,
                       double* valid, double* T_int, double* t_final, void* stream) {
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
-                       workspace, lib, valid, T_int, t_final, nullptr, 0, stream);
+                       workspace, lib, valid, T_int, t_final, nullptr, 0,
+                       knob(KN_LIB_BINNING) == 1, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -708,7 +932,7 @@ int smx_build_library_noalloc(const smx_potential* p, const smx_grids* g, const 
   double* ws = nullptr;
   if (int e = scratch_for_current_device(&ws)) return e;
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp, ws,
-                       lib, valid, T_int, t_final, nullptr, 0, stream);
+                       lib, valid, T_int, t_final, nullptr, 0, false, stream);
 }
 
 int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
@@ -719,7 +943,7 @@ int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx
                            int32_t ckpt_every, void* stream) {
   SMX_REQUIRE(ckpt != nullptr, "smx_build_library_ckpt: null checkpoint buffer");
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
-                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every, stream);
+                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every, false, stream);
 }
 
 int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
@@ -749,8 +973,9 @@ int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx
                      s->N_max, o_traj, t_traj, T_int, bin_mapping, v0, sdisp, ws);
   SMX_LAUNCHED();
   const int64_t items = (int64_t)P + g->num_vbin;   // finalize work items per column
-  hipLaunchKernelGGL(finalize_library_kernel, dim3(blocks_for(n * items, 256)), dim3(256), 0, st, ws,
-                     n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, 1, lib);
+  hipLaunchKernelGGL(finalize_library_kernel<false>, dim3(blocks_for(n * items, 256)), dim3(256), 0,
+                     st, ws, n, r, P, g->num_vbin, valid, T_int, num_per_bin, g->area_pixel, dt0, *s,
+                     1, sdisp, lib);
   SMX_LAUNCHED();
   return 0;
 }

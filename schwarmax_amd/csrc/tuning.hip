@@ -12,9 +12,10 @@ namespace {
 // plain-20k shapes), syrkx from 16,384 rows, the thin FP64 GEMM, and a
 // split-K target of 2,048 workgroups for the tiled FP64 GEMM (the right-hand
 // sides and one anchor: 5.9 ms against 6.5 ms at 1,024 at plain-20k shapes).
-constexpr int64_t kDefault[KN_COUNT] = {2, 8, 16384, 2048, 0, 1};
+constexpr int64_t kDefault[KN_COUNT] = {2, 8, 16384, 2048, 0, 1, 1};
 std::atomic<int64_t> g_knob[KN_COUNT] = {{kDefault[0]}, {kDefault[1]}, {kDefault[2]},
-                                         {kDefault[3]}, {kDefault[4]}, {kDefault[5]}};
+                                         {kDefault[3]}, {kDefault[4]}, {kDefault[5]},
+                                         {kDefault[6]}};
 
 bool valid(int k, int64_t v) {
   switch (k) {
@@ -24,6 +25,7 @@ bool valid(int k, int64_t v) {
     case KN_GEMM64_SPLIT: return v >= 256 && v <= 4096;
     case KN_ADMM_LOADS: return v >= 0 && v <= 2;
     case KN_GEMM64_KERNEL: return v == 1 || v == 2;
+    case KN_LIB_BINNING: return v == 1 || v == 2;
     default: return false;
   }
 }

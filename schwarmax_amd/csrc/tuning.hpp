@@ -14,6 +14,7 @@ enum Knob {
   KN_GEMM64_SPLIT,       // workgroups the FP64 GEMM's split-K rule fills (256..4096)
   KN_ADMM_LOADS,         // ADMM product's loads of M: 0 auto, 1 non-temporal, 2 cached
   KN_GEMM64_KERNEL,      // FP64 GEMMs with few rows: 1 thin kernel, 2 tiled kernel with split-K
+  KN_LIB_BINNING,        // smx_build_library: 1 samples + LDS histograms, 2 fused integrator (same bits)
   KN_COUNT
 };
 

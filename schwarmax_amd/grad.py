@@ -157,7 +157,7 @@ class _Library(torch.autograd.Function):
         bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
         npb = _lib.dev(dict_data['num_per_bin'])
         v0 = _lib.dev(dict_data['v0'])
-        sd = _lib.dev(dict_data['s'])
+        sd = _lib.dispersions(dict_data['s'])
         wb = wb.contiguous()
         dt0 = dt0.contiguous()
         _lib.check(L.smx_build_library_ckpt(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),

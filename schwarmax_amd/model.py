@@ -134,14 +134,16 @@ def build_library(pot, grids, sched_struct, wb, Tt, dt0, dict_data, traj=False):
     n_int = n * r
     P = grids.nR * grids.nz * grids.nphi
     B = grids.num_vbin
-    ws = torch.empty(int(L.smx_library_workspace_bytes(grids, n_int)), dtype=torch.uint8, device='cuda')
+    ws_bytes = (L.smx_library_workspace_bytes(grids, n_int) if traj
+                else L.smx_library_build_bytes(grids, sched_struct, n_int))
+    ws = torch.empty(int(ws_bytes), dtype=torch.uint8, device='cuda')
     lib = torch.empty((n, P + 6 * B), dtype=torch.float64, device='cuda')
     valid = torch.empty(n_int, dtype=torch.float64, device='cuda')
     T_int = torch.empty(n_int, dtype=torch.float64, device='cuda')
     bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
     npb = _lib.dev(dict_data['num_per_bin'])
     v0 = _lib.dev(dict_data['v0'])
-    sd = _lib.dev(dict_data['s'])
+    sd = _lib.dispersions(dict_data['s'])
     extra = {}
     if traj:
         N = sched_struct.N_max

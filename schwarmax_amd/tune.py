@@ -45,6 +45,7 @@ KNOB_INV_SYRKX_MIN = 2
 KNOB_GEMM64_SPLIT_TARGET = 3
 KNOB_ADMM_LOADS = 4
 KNOB_GEMM64_KERNEL = 5
+KNOB_LIB_BINNING = 6
 GRAM_WAVES = (4, 8)
 GEMM64_KERNELS = (1, 2)   # 1 thin kernel, 2 tiled kernel with split-K
 SYRKX_MIN = (4096, 8192, 16384, 32768)

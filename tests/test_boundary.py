@@ -69,6 +69,13 @@ def test_workspace_queries_without_gpu():
                      (20, 20), np.eye(3), True)
     P = 8 * 4 * 6
     assert L.smx_library_workspace_bytes(g, 10) == 10 * 8 * (((P + 7) // 8) * 8 + 6 * 100)
+    # smx_build_library: the larger of the slabs and the sample rows (64 bytes
+    # per attempt)
+    slabs = 10 * 8 * (((P + 7) // 8) * 8 + 6 * 100)
+    for N_max in (5, 50, 5000):
+        sc = _lib.SmxSchedule(N_max=N_max, Omega=0.0, atol=1e-6, rtol=1e-4, dt_min=1e-5, dt_max=0.3)
+        assert L.smx_library_build_bytes(g, sc, 10) == max(slabs, 10 * N_max * 64)
+    assert L.smx_library_build_bytes(g, sc, -1) == -1
     assert L.smx_nnls_workspace_bytes(1000, 700, 20) > 4 * 1000 * 1000
     assert L.smx_nnls_workspace_bytes(0, 1, 1) == -1
 
@@ -152,11 +159,13 @@ def test_tuning_knobs_without_gpu():
     v = ctypes.c_int64(0)
     defaults = {tune.KNOB_ADMM_OCCUPANCY: 2, tune.KNOB_GRAM_WAVES: 8,
                 tune.KNOB_INV_SYRKX_MIN: 16384, tune.KNOB_GEMM64_SPLIT_TARGET: 2048,
-                tune.KNOB_ADMM_LOADS: 0, tune.KNOB_GEMM64_KERNEL: 1}
+                tune.KNOB_ADMM_LOADS: 0, tune.KNOB_GEMM64_KERNEL: 1, tune.KNOB_LIB_BINNING: 1}
     bad = {tune.KNOB_ADMM_OCCUPANCY: 9, tune.KNOB_GRAM_WAVES: 6, tune.KNOB_INV_SYRKX_MIN: 100,
-           tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3, tune.KNOB_GEMM64_KERNEL: 3}
+           tune.KNOB_GEMM64_SPLIT_TARGET: 8192, tune.KNOB_ADMM_LOADS: 3, tune.KNOB_GEMM64_KERNEL: 3,
+           tune.KNOB_LIB_BINNING: 3}
     good = {tune.KNOB_ADMM_OCCUPANCY: 3, tune.KNOB_GRAM_WAVES: 4, tune.KNOB_INV_SYRKX_MIN: 4096,
-            tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2, tune.KNOB_GEMM64_KERNEL: 2}
+            tune.KNOB_GEMM64_SPLIT_TARGET: 512, tune.KNOB_ADMM_LOADS: 2, tune.KNOB_GEMM64_KERNEL: 2,
+            tune.KNOB_LIB_BINNING: 2}
     for k in defaults:
         assert L.smx_tune_get(k, ctypes.byref(v)) == 0
         before = v.value

@@ -1,7 +1,9 @@
 """smx_build_library_noalloc (the orbit library without device allocation)
 against smx_build_library on the same inputs: two calls in a row, the second
 smaller, so that the shared scratch array's leftovers from the first call
-would show.  Same kernels on the same inputs: the results must be identical.
+would show.  The no-allocation entry point bins in the fused integrator's
+slabs, smx_build_library stores samples and bins them in LDS histograms; both
+hold the sums as 64-bit fixed point, so the results must be identical.
 A call whose slabs exceed the scratch array is refused before any launch."""
 import ctypes
 
@@ -52,7 +54,7 @@ def _run(noalloc, args):
         rc = L.smx_build_library_noalloc(*common, *outs, _lib.stream_ptr())
         _lib.check(rc, 'smx_build_library_noalloc')
     else:
-        ws = torch.empty(int(L.smx_library_workspace_bytes(g, n * r)), dtype=torch.uint8,
+        ws = torch.empty(int(L.smx_library_build_bytes(g, sc, n * r)), dtype=torch.uint8,
                          device='cuda')
         _lib.check(L.smx_build_library(*common, _lib.ptr(ws), *outs, _lib.stream_ptr()),
                    'smx_build_library')

@@ -415,12 +415,26 @@ __global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
     const double* __restrict__ dt0, const double* __restrict__ valid,
     const double* __restrict__ T_int, const int32_t* __restrict__ bm,
     const double* __restrict__ v0, const double* __restrict__ sd,
-    const double* __restrict__ num_per_bin, int64_t win, double* __restrict__ lib) {
+    const double* __restrict__ num_per_bin, int64_t win, int stage, double* __restrict__ lib) {
   extern __shared__ long long hist[];
   const int64_t c = blockIdx.x;
   const int tid = threadIdx.x;
   const int P = g.nR * g.nz * g.nphi;
   const int B = g.num_vbin;
+  // the apertures' v0 and s in LDS after the window when they fit: the
+  // binning's gathers of them then stay on the CU
+  const double* v0p = v0;
+  const double* sdp = sd;
+  if (stage) {
+    double* sv = reinterpret_cast<double*>(hist + win);
+    for (int a = tid; a < B; a += BL_THREADS) {
+      sv[a] = v0[a];
+      sv[B + a] = sd[a];
+    }
+    __syncthreads();
+    v0p = sv;
+    sdp = sv + B;
+  }
   const int64_t rows = (int64_t)P + 6 * (int64_t)B;
   const int64_t N = s.N_max;
   const int64_t Ls = (N + 63) / 64;
@@ -441,37 +455,51 @@ __global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
       const int e_t = fix_exp_time(dt0[it], s, NC);
       const double sc_t = fix_scale(e_t);
       const double* base = samples + it * N * kSampleW;
-      for (int64_t j = wave; j < Ls; j += BL_THREADS / 64) {
-        const int64_t a_idx = lane * Ls + j;
-        if (a_idx >= N) continue;
-        const double* row = base + a_idx * kSampleW;
-        const double2 y01 = *reinterpret_cast<const double2*>(row);
-        const double2 y23 = *reinterpret_cast<const double2*>(row + 2);
-        const double2 y45 = *reinterpret_cast<const double2*>(row + 4);
-        const double used = row[6];
-        if (used == 0.0) continue;
-        const double y[6] = {y01.x, y01.y, y23.x, y23.y, y45.x, y45.y};
-        SampleProj sp;
-        project(g, y, NC > 2, sp);
-        const long long qd = fix_q(used, sc_t);
-#pragma unroll 1
-        for (int cp = 0; cp < NC; ++cp) {
-          int cell, a;
-          long long q[kApRec];
-          copy_bins(g, bm, v0, sd, sp, y[2], cp, used, e_t, cell, a, q);
-          if (cell >= e0 && cell < e1)
-            atomicAdd(reinterpret_cast<unsigned long long*>(&hist[cell - e0]),
-                      static_cast<unsigned long long>(qd));
-          if (a >= 0) {
+      // attempts l Ls + j of this lane, the next one's row loaded while the
+      // current one is binned
+      constexpr int WV = BL_THREADS / 64;
+      double2 cur[4], nxt[4];
+      auto load_row = [&](int64_t jj, double2 (&r)[4]) {
+        const int64_t a_idx = lane * Ls + jj;
+        if (a_idx < N) {
+          const double2* row = reinterpret_cast<const double2*>(base + a_idx * kSampleW);
 #pragma unroll
-            for (int comp = 0; comp < kApRec; ++comp) {
-              const int64_t e = P + rec_block(comp) * (int64_t)B + a;
-              if (e >= e0 && e < e1)
-                atomicAdd(reinterpret_cast<unsigned long long*>(&hist[e - e0]),
-                          static_cast<unsigned long long>(q[comp]));
+          for (int h = 0; h < 4; ++h) r[h] = row[h];
+        } else {
+          r[3] = make_double2(0.0, 0.0);   // no attempt: step length 0
+        }
+      };
+      int64_t j = wave;
+      if (j < Ls) load_row(j, cur);
+      for (; j < Ls; j += WV) {   // wave-uniform
+        if (j + WV < Ls) load_row(j + WV, nxt);
+        const double used = cur[3].x;
+        if (used != 0.0) {
+          const double y[6] = {cur[0].x, cur[0].y, cur[1].x, cur[1].y, cur[2].x, cur[2].y};
+          SampleProj sp;
+          project(g, y, NC > 2, sp);
+          const long long qd = fix_q(used, sc_t);
+#pragma unroll
+          for (int cp = 0; cp < NC; ++cp) {
+            int cell, a;
+            long long q[kApRec];
+            copy_bins(g, bm, v0p, sdp, sp, y[2], cp, used, e_t, cell, a, q);
+            if (cell >= e0 && cell < e1)
+              atomicAdd(reinterpret_cast<unsigned long long*>(&hist[cell - e0]),
+                        static_cast<unsigned long long>(qd));
+            if (a >= 0) {
+#pragma unroll
+              for (int comp = 0; comp < kApRec; ++comp) {
+                const int64_t e = P + rec_block(comp) * (int64_t)B + a;
+                if (e >= e0 && e < e1)
+                  atomicAdd(reinterpret_cast<unsigned long long*>(&hist[e - e0]),
+                            static_cast<unsigned long long>(q[comp]));
+              }
             }
           }
         }
+#pragma unroll
+        for (int h = 0; h < 4; ++h) cur[h] = nxt[h];
       }
       __syncthreads();
       const double T = T_int[it];
@@ -484,13 +512,16 @@ __global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
         int off = tid + q * BL_THREADS;
         asm volatile("" : "+v"(off));
         const int64_t e = e0 + off;
-        if (e < e1) {
-          const double S = static_cast<double>(hist[e - e0]);
+        // an empty entry adds +0 (S = 0 gives v = +0, and the average is
+        // never -0), so it is skipped: most cells see no sample of one orbit
+        const long long Sq = e < e1 ? hist[e - e0] : 0;
+        if (Sq != 0) {
+          const double S = static_cast<double>(Sq);
           if (e < P) {
             lib_add(out[q], S * un_t, T, false, 1.0, true, wt);
           } else {
             const int64_t blk = (e - P) / B, a = (e - P) - blk * B;
-            const double un = blk == 0 ? un_t : fix_unscale(e_t + fix_exp_gh(sd[a]));
+            const double un = blk == 0 ? un_t : fix_unscale(e_t + fix_exp_gh(sdp[a]));
             lib_add(out[q], S * un, T, blk == 0, aperture_area(num_per_bin[a], g.area_pixel), true, wt);
           }
         }
@@ -782,7 +813,7 @@ static int check_grids(const smx_grids* g) {
 
 // Rows per bin_library_kernel window: the whole row set when it fits the
 // workgroup's LDS, else as many as fit (at most BL_THREADS x BL_PER_THREAD).
-static int library_window(int64_t rows, int64_t* win) {
+static int library_window(int64_t rows, int64_t B, int64_t* win, int* stage, size_t* bytes) {
   static int lds = 0;
   static std::mutex mu;
   std::lock_guard<std::mutex> lock(mu);
@@ -791,17 +822,18 @@ static int library_window(int64_t rows, int64_t* win) {
     SMX_HIP(hipGetDevice(&dev));
     SMX_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     SMX_REQUIRE(v >= 8 * BL_THREADS, "bin_library_kernel: %d bytes of LDS per workgroup", v);
-    lds = v;
-    const int64_t cap = (int64_t)BL_THREADS * BL_PER_THREAD * (int64_t)sizeof(long long);
-    const int bytes = (int)(lds < cap ? lds : cap);
     SMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_library_kernel<1>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v));
     SMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(bin_library_kernel<4>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v));
+    lds = v;
   }
   int64_t cap = lds / (int64_t)sizeof(long long);
   if (cap > (int64_t)BL_THREADS * BL_PER_THREAD) cap = (int64_t)BL_THREADS * BL_PER_THREAD;
   *win = rows < cap ? rows : cap;
+  const int64_t need = (*win + 2 * B) * (int64_t)sizeof(double);
+  *stage = need <= lds ? 1 : 0;
+  *bytes = (size_t)(*stage ? need : *win * (int64_t)sizeof(long long));
   return 0;
 }
 
@@ -832,7 +864,9 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
   hipStream_t st = as_stream(stream);
   if (samples) {
     int64_t win = 0;
-    if (int e = library_window((int64_t)P + 6 * (int64_t)B, &win)) return e;
+    int stage = 0;
+    size_t lds_bytes = 0;
+    if (int e = library_window((int64_t)P + 6 * (int64_t)B, B, &win, &stage, &lds_bytes)) return e;
     double* smp = static_cast<double*>(workspace);
     stage_begin(ST_INTEGRATE, st);
     hipLaunchKernelGGL(integrate_library_kernel<true>, dim3((unsigned)integrator_waves<true>(n_int)),
@@ -842,13 +876,13 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
     stage_end(ST_INTEGRATE, st, 1);
     stage_begin(ST_FINALIZE, st);
     if (NC == 4)
-      hipLaunchKernelGGL(bin_library_kernel<4>, dim3((unsigned)n), dim3(BL_THREADS),
-                         win * sizeof(long long), st, *g, *s, smp, n, (int)r, dt0, valid, T_int,
-                         bin_mapping, v0, sdisp, num_per_bin, win, lib);
+      hipLaunchKernelGGL(bin_library_kernel<4>, dim3((unsigned)n), dim3(BL_THREADS), lds_bytes, st,
+                         *g, *s, smp, n, (int)r, dt0, valid, T_int, bin_mapping, v0, sdisp,
+                         num_per_bin, win, stage, lib);
     else
-      hipLaunchKernelGGL(bin_library_kernel<1>, dim3((unsigned)n), dim3(BL_THREADS),
-                         win * sizeof(long long), st, *g, *s, smp, n, (int)r, dt0, valid, T_int,
-                         bin_mapping, v0, sdisp, num_per_bin, win, lib);
+      hipLaunchKernelGGL(bin_library_kernel<1>, dim3((unsigned)n), dim3(BL_THREADS), lds_bytes, st,
+                         *g, *s, smp, n, (int)r, dt0, valid, T_int, bin_mapping, v0, sdisp,
+                         num_per_bin, win, stage, lib);
     SMX_LAUNCHED();
     stage_end(ST_FINALIZE, st, 1);
     return 0;

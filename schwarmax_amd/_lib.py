@@ -236,6 +236,11 @@ def stream_ptr(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+def _host(a):
+    h = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return np.asarray(h, dtype=np.float64).reshape(-1)
+
+
 def dispersions(s, device='cuda'):
     """The apertures' Gauss-Hermite dispersions s as a device FP64 tensor.
 
@@ -243,10 +248,19 @@ def dispersions(s, device='cuda'):
     point scaled by a bound proportional to 1/s, so every s must be positive
     and finite; the reference's gh weights (ghmoments.py:6-36) would be NaN or
     infinite otherwise.  Raises ValueError."""
-    h = s.detach().cpu().numpy() if isinstance(s, torch.Tensor) else np.asarray(s)
-    h = np.asarray(h, dtype=np.float64).reshape(-1)
+    h = _host(s)
     if not np.all(np.isfinite(h) & (h > 0.0)):
         raise ValueError("aperture dispersions 's' must be positive and finite")
+    return dev(h, device=device)
+
+
+def velocities(v0, device='cuda'):
+    """The apertures' Gauss-Hermite mean velocities v0 as a device FP64 tensor;
+    they must be finite, as the fixed-point binned terms cannot carry a NaN.
+    Raises ValueError."""
+    h = _host(v0)
+    if not np.all(np.isfinite(h)):
+        raise ValueError("aperture velocities 'v0' must be finite")
     return dev(h, device=device)
 
 

@@ -119,7 +119,7 @@ def model_likelihood_batch(thetas, dict_data, num_Vbin, Rzphi_n_tot, n_orbits=No
     normals, uni = (dev(a) for a in MD.ic_draws(dict_data, r))
     r_total = uni.shape[1]
     bm = _lib.bin_map(dict_data['bin_mapping'], grids[0].nX, grids[0].nY)
-    npb, v0, sd = dev(dict_data['num_per_bin']), dev(dict_data['v0']), _lib.dispersions(dict_data['s'])
+    npb, v0, sd = dev(dict_data['num_per_bin']), _lib.velocities(dict_data['v0']), _lib.dispersions(dict_data['s'])
 
     main = torch.cuda.current_stream()
     side = torch.cuda.Stream()

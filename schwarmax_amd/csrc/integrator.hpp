@@ -165,8 +165,13 @@ __device__ __forceinline__ int fix_exp_gh(double s) { return fix_exp_of(2.0 * kI
 __device__ __forceinline__ double fix_scale(int e) { return ldexp(1.0, kFixBits - e); }
 __device__ __forceinline__ double fix_unscale(int e) { return ldexp(1.0, e - kFixBits); }
 
+// Rounded multiple; a value off the bound (only from an integration that
+// diverged, which the validity gate then drops) saturates at +-2^62 and NaN
+// maps to 0, so the conversion stays defined.
 __device__ __forceinline__ long long fix_q(double x, double scale) {
-  return static_cast<long long>(rint(x * scale));
+  const double v = rint(x * scale);
+  const double lim = 4611686018427387904.0;   // 2^62
+  return static_cast<long long>(v < lim ? (v > -lim ? v : -lim) : (v == v ? lim : 0.0));
 }
 
 // Area of an aperture of npb pixels (integrates.py:212-236), one rounding

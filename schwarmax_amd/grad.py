@@ -156,7 +156,7 @@ class _Library(torch.autograd.Function):
         ckpt = torch.empty((max(nck, 1), n_int, 8), dtype=torch.float64, device='cuda')
         bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
         npb = _lib.dev(dict_data['num_per_bin'])
-        v0 = _lib.dev(dict_data['v0'])
+        v0 = _lib.velocities(dict_data['v0'])
         sd = _lib.dispersions(dict_data['s'])
         wb = wb.contiguous()
         dt0 = dt0.contiguous()

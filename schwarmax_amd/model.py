@@ -142,7 +142,7 @@ def build_library(pot, grids, sched_struct, wb, Tt, dt0, dict_data, traj=False):
     T_int = torch.empty(n_int, dtype=torch.float64, device='cuda')
     bm = _lib.bin_map(dict_data['bin_mapping'], grids.nX, grids.nY)
     npb = _lib.dev(dict_data['num_per_bin'])
-    v0 = _lib.dev(dict_data['v0'])
+    v0 = _lib.velocities(dict_data['v0'])
     sd = _lib.dispersions(dict_data['s'])
     extra = {}
     if traj:

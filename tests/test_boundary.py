@@ -115,6 +115,27 @@ def test_pointer_dtype_is_checked():
     assert got.dtype == torch.float64 and np.array_equal(got.numpy(), np.arange(5.0))
 
 
+def test_aperture_moments_are_validated():
+    """The binned Gauss-Hermite terms are fixed point scaled by 1/s: the host
+    refuses non-positive or non-finite dispersions and non-finite v0 before
+    any launch, and passes good ones through as float64."""
+    import pytest
+    from schwarmax_amd import _lib
+    good = _lib.dispersions(np.array([10.0, 150.0], dtype=np.float32), device='cpu')
+    assert good.dtype == torch_float64() and np.array_equal(good.numpy(), [10.0, 150.0])
+    for bad in ([10.0, 0.0], [10.0, -1.0], [np.nan, 5.0], [np.inf, 5.0]):
+        with pytest.raises(ValueError, match="dispersions"):
+            _lib.dispersions(np.array(bad), device='cpu')
+    assert np.array_equal(_lib.velocities([-30.0, 0.0, 12.5], device='cpu').numpy(), [-30.0, 0.0, 12.5])
+    with pytest.raises(ValueError, match="v0"):
+        _lib.velocities([0.0, np.nan], device='cpu')
+
+
+def torch_float64():
+    import torch
+    return torch.float64
+
+
 def test_bin_map_with_and_without_sentinel():
     """Off-field samples read entry nX*nY: the sentinel when present; without
     it, JAX's clamped gather of index nX*nY, i.e. the last pixel's aperture."""

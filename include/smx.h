@@ -347,6 +347,44 @@ int smx_nnls_admm_group(int32_t n_models, const smx_nnls_problem* problems, int6
 int smx_nnls_group_full_products(int64_t n, int64_t m, int64_t K, const void* state,
                                  int32_t* full_products, void* stream);
 
+/* ---- model maps epilogue and bootstrap chi^2 (forward path) ------------- */
+/* From maps (K, ldm) = W lib[:, :m] with rows [Rzphi (P) | xy (B) | h1..h4
+ * (B each)]: surface_density (K, B) = (map_xy * mmpo) * lm, h (4, K, B) =
+ * clip(map_h / gamma, -10, 10) with |gamma| floored at 1e-8, and V, sigma
+ * (K, B) from h (model.py:14-66, 260-286; ghmoments.py:53-79). */
+int smx_model_epilogue(const double* maps, int64_t ldm, int64_t K, int32_t P, int32_t B,
+                       const double* gamma_kin, const double* v0, const double* s, double mmpo,
+                       double lm, double* surface_density, double* h, double* V, double* sigma,
+                       void* stream);
+/* Bootstrap data of the chi^2: XY_density_data, its error and the (K, B)
+ * standard normals; h1..h4 data, errors and normals (likelihood.py:184-244). */
+typedef struct smx_bootstrap_data {
+  const double* xy;
+  const double* xy_err;
+  const double* xy_normal;
+  const double* h[4];
+  const double* h_err[4];
+  const double* h_normal[4];
+} smx_bootstrap_data;
+/* The model side of the chi^2: density_3d_model d3 (K rows, leading
+ * dimension ld3 >= P), the density target y_R (P), surface_density and
+ * h1..h4 (K, B each, contiguous). */
+typedef struct smx_bootstrap_model {
+  const double* d3;
+  int64_t ld3;
+  const double* y_R;
+  const double* surface_density;
+  const double* h[4];
+} smx_bootstrap_model;
+/* Bootstrap log-likelihood rows logl_rows (K) and their log-mean-exp logl (1)
+ * (likelihood.py:184-244): every row adds the density term summed over all
+ * rows and cells (the reference broadcasts res_Rzphi), NaN terms skipped.
+ * work: 2K doubles.  The forward path's chi^2; the gradient takes its chi^2
+ * through torch autograd. */
+int smx_bootstrap_loglike(const smx_bootstrap_model* model, int64_t K, int32_t P, int32_t B,
+                          const smx_bootstrap_data* data, double sigma_amplify, double* work,
+                          double* logl_rows, double* logl, void* stream);
+
 /* ---- ADMM product tuning (no reference counterpart: the knob the
  * autotuner in schwarmax_amd/tune.py sets) ------------------------------- */
 /* 256-thread workgroups per CU that the ADMM product's split-K rule targets

@@ -50,6 +50,17 @@ class SmxSchedule(ctypes.Structure):
     ]
 
 
+class SmxBootstrapData(ctypes.Structure):
+    _fields_ = [('xy', ctypes.c_void_p), ('xy_err', ctypes.c_void_p), ('xy_normal', ctypes.c_void_p),
+                ('h', ctypes.c_void_p * 4), ('h_err', ctypes.c_void_p * 4),
+                ('h_normal', ctypes.c_void_p * 4)]
+
+
+class SmxBootstrapModel(ctypes.Structure):
+    _fields_ = [('d3', ctypes.c_void_p), ('ld3', ctypes.c_int64), ('y_R', ctypes.c_void_p),
+                ('surface_density', ctypes.c_void_p), ('h', ctypes.c_void_p * 4)]
+
+
 class SmxNnlsProblem(ctypes.Structure):
     _fields_ = [
         ('lib', ctypes.c_void_p), ('ld_lib', ctypes.c_int64), ('n', ctypes.c_int64),
@@ -153,6 +164,11 @@ _SIGS = [
     ('smx_gemm_nt_f64_split', ctypes.c_int, [_I64, _I64, _I64, _D, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P]),
     ('smx_spd_inverse_f64', ctypes.c_int, [_P, _I64, _P, _P]),
     ('smx_model_maps', ctypes.c_int, [_P, _I64, _I64, _I64, _P, _I64, _P, _P]),
+    ('smx_model_epilogue', ctypes.c_int,
+     [_P, _I64, _I64, _I32, _I32, _P, _P, _P, _D, _D, _P, _P, _P, _P, _P]),
+    ('smx_bootstrap_loglike', ctypes.c_int,
+     [ctypes.POINTER(SmxBootstrapModel), _I64, _I32, _I32, ctypes.POINTER(SmxBootstrapData), _D,
+      _P, _P, _P, _P]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -66,7 +66,56 @@ def chi2_func(y_Rzphi, y_Rzphi_model, sigma_Rzphi,
 
 
 def bootstrap_loglike(model_dict, dict_data, sigma_amplify=1.0):
-    """Bootstrap chi^2 rows and their log-mean-exp (likelihood.py:184-244)."""
+    """Bootstrap chi^2 rows and their log-mean-exp (likelihood.py:184-244).
+
+    Forward evaluations run smx_bootstrap_loglike; when the model maps carry
+    an autograd graph (the gradient), the same formula runs as torch
+    operations so that autograd can differentiate it."""
+    if not (torch.is_grad_enabled() and any(
+            isinstance(model_dict[k], torch.Tensor) and model_dict[k].requires_grad
+            for k in ('density_3d', 'density_3d_model', 'surface_density', 'h1', 'h2', 'h3', 'h4'))):
+        return _bootstrap_loglike_hip(model_dict, dict_data, sigma_amplify)
+    return _bootstrap_loglike_graph(model_dict, dict_data, sigma_amplify)
+
+
+def _bootstrap_loglike_hip(model_dict, dict_data, sigma_amplify):
+    from . import _lib
+    L = _lib.lib()
+    f64 = torch.float64
+    keep = []
+
+    def dptr(t):
+        t = _lib.dev(t)
+        keep.append(t)
+        return _lib.ptr(t, f64).value
+
+    d3 = _lib.dev(model_dict['density_3d_model'])
+    if d3.dim() != 2 or d3.stride(1) != 1:
+        d3 = d3.contiguous()
+    keep.append(d3)
+    K, P = d3.shape
+    d2 = _lib.dev(model_dict['surface_density'])
+    B = d2.shape[1]
+    model = _lib.SmxBootstrapModel(d3=d3.data_ptr(), ld3=d3.stride(0),
+                                   y_R=dptr(model_dict['density_3d']), surface_density=dptr(d2))
+    data = _lib.SmxBootstrapData(xy=dptr(dict_data['XY_density_data']),
+                                 xy_err=dptr(dict_data['XY_density_data_err']),
+                                 xy_normal=dptr(dict_data['XY_standard_normal']))
+    for j in range(4):
+        model.h[j] = dptr(model_dict['h%d' % (j + 1)])
+        data.h[j] = dptr(dict_data['h%d_data' % (j + 1)])
+        data.h_err[j] = dptr(dict_data['h%d_data_err' % (j + 1)])
+        data.h_normal[j] = dptr(dict_data['h%d_standard_normal' % (j + 1)])
+    work = torch.empty(2 * K, dtype=f64, device='cuda')
+    rows = torch.empty(K, dtype=f64, device='cuda')
+    out = torch.empty((), dtype=f64, device='cuda')
+    _lib.check(L.smx_bootstrap_loglike(model, K, P, B, data, float(sigma_amplify), _lib.ptr(work, f64),
+                                       _lib.ptr(rows, f64), _lib.ptr(out, f64), _lib.stream_ptr()),
+               'smx_bootstrap_loglike')
+    return out, rows
+
+
+def _bootstrap_loglike_graph(model_dict, dict_data, sigma_amplify):
     from ._lib import dev
     y_R = model_dict['density_3d']
     y_xy = dev(dict_data['XY_density_data'])

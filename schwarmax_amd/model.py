@@ -20,7 +20,6 @@ from . import _lib
 from . import potentials as POT
 from . import prng
 from .utils import makeRotationMatrix
-from .ghmoments import h_to_V_sigma
 
 EPSILON = 1e-12
 DEFAULT_SCHEDULE = dict(N_step_per_orb=100, N_dynamical_time=50, N_max=None, atol=1e-7,
@@ -184,17 +183,27 @@ def density_target(params_disk_rho, dict_data):
     return out
 
 
-def compute_model_single(W, lib, P, B, gamma_kin, v0, s):
-    """Model maps for weight rows W (K, n) (model.py:14-66)."""
+def compute_model_single(W, lib, P, B, gamma_kin, v0, s, mmpo=1.0, lm=1.0):
+    """Model maps for weight rows W (K, n) (model.py:14-66): one FP64 GEMM
+    (smx_model_maps), then smx_model_epilogue for the surface density
+    (map * mmpo) * lm, the clamped h1..h4 and V, sigma (ghmoments.py:53-79).
+    d3 is a view of the maps."""
     from .nnls import model_maps
+    L = _lib.lib()
     m = P + 5 * B
     maps = model_maps(lib, m, W)
-    g = torch.where(gamma_kin.abs() > 1e-8, gamma_kin, torch.full_like(gamma_kin, 1e-8))
-    d3 = maps[:, :P]
-    d2 = maps[:, P:P + B]
-    h = [torch.clamp(maps[:, P + B * (k + 1):P + B * (k + 2)] / g, -10.0, 10.0) for k in range(4)]
-    V, sig = h_to_V_sigma(h[0], h[1], v0, s, h3=h[2], h4=h[3])
-    return d3, d2, h[0], h[1], h[2], h[3], V, sig
+    K = maps.shape[0]
+    f64 = torch.float64
+    d2 = torch.empty((K, B), dtype=f64, device='cuda')
+    h = torch.empty((4, K, B), dtype=f64, device='cuda')
+    V = torch.empty((K, B), dtype=f64, device='cuda')
+    sig = torch.empty((K, B), dtype=f64, device='cuda')
+    g, v0, s = (_lib.dev(t) for t in (gamma_kin, v0, s))
+    _lib.check(L.smx_model_epilogue(_lib.ptr(maps, f64), m, K, P, B, _lib.ptr(g, f64),
+                                    _lib.ptr(v0, f64), _lib.ptr(s, f64), float(mmpo), float(lm),
+                                    _lib.ptr(d2, f64), _lib.ptr(h, f64), _lib.ptr(V, f64),
+                                    _lib.ptr(sig, f64), _lib.stream_ptr()), 'smx_model_epilogue')
+    return maps[:, :P], d2, h[0], h[1], h[2], h[3], V, sig
 
 
 def _forward(density_func, potential_func, params_halo_pot, params_disk_rho, dict_data, num_Vbin,
@@ -264,8 +273,7 @@ def _bootstrap_data(n, params_disk_rho, dict_data):
 def _model_maps(W, lib, P, B, t):
     """The model dict of model.py:260-286 from the weights."""
     d3, d2, h1, h2, h3, h4, V, sig = compute_model_single(W, lib, P, B, t['gamma_kin'], t['v0_ap'],
-                                                          t['s_ap'])
-    d2 = d2 * t['mmpo'] * t['lm']
+                                                          t['s_ap'], t['mmpo'], t['lm'])
     return {'density_3d': t['y_R'], 'density_3d_model': d3, 'surface_density': d2, 'h1': h1,
             'h2': h2, 'h3': h3, 'h4': h4, 'V': V, 'sigma': sig, 'weights': W}
 

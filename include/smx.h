@@ -105,8 +105,11 @@ int smx_initial_conditions(const smx_potential* p, const double* w0, const doubl
  * integrates.py:132-143, utils.py:429).  Ids outside [0, num_vbin) are
  * dropped, as segment_sum drops them. */
 /* Bytes of the per-integration accumulator slabs of n_int integrations: the
- * workspace of smx_build_library_ckpt and smx_build_library_traj. */
+ * workspace of smx_build_library_traj and smx_build_library_noalloc's limit. */
 int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int);
+/* Bytes of the workspace of smx_build_library_ckpt: the slabs, which the
+ * adjoint reads, followed by the sample rows. */
+int64_t smx_library_ckpt_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int);
 /* Bytes of the workspace of smx_build_library for n_int integrations: the
  * larger of the slabs and the sample rows (N_max 64-byte rows each). */
 int64_t smx_library_build_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int);
@@ -193,8 +196,10 @@ int smx_density_target(const smx_potential* p, const double* R_grid, const doubl
 
 /* smx_build_library that also stores, for the adjoint, the state
  * ckpt[(j, i, 0:8)] = (y[6], dt, t) of integration i before attempt
- * j * ckpt_every, j < ceil(N_max / ckpt_every).  The slabs in `workspace`
- * stay valid until the caller reuses it; smx_library_vjp reads them. */
+ * j * ckpt_every, j < ceil(N_max / ckpt_every), and keeps each valid
+ * integration's fixed-point sums in the slabs at the start of `workspace`
+ * (smx_library_ckpt_bytes; the sample rows follow them).  The slabs stay
+ * valid until the caller reuses the workspace; smx_library_vjp reads them. */
 int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                            const double* wb, const double* T_total, const double* dt0,
                            int64_t n, int32_t r, const int32_t* bin_mapping,

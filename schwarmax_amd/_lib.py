@@ -99,6 +99,8 @@ _SIGS = [
     ('smx_library_workspace_bytes', _I64, [ctypes.POINTER(SmxGrids), _I64]),
     ('smx_library_build_bytes', _I64,
      [ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule), _I64]),
+    ('smx_library_ckpt_bytes', _I64,
+     [ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule), _I64]),
     ('smx_build_library', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

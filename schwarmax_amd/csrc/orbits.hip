@@ -415,7 +415,8 @@ __global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
     const double* __restrict__ dt0, const double* __restrict__ valid,
     const double* __restrict__ T_int, const int32_t* __restrict__ bm,
     const double* __restrict__ v0, const double* __restrict__ sd,
-    const double* __restrict__ num_per_bin, int64_t win, int stage, double* __restrict__ lib) {
+    const double* __restrict__ num_per_bin, int64_t win, int stage, double* __restrict__ lib,
+    long long* __restrict__ slabs) {
   extern __shared__ long long hist[];
   const int64_t c = blockIdx.x;
   const int tid = threadIdx.x;
@@ -502,6 +503,20 @@ __global__ __launch_bounds__(BL_THREADS) void bin_library_kernel(
         for (int h = 0; h < 4; ++h) cur[h] = nxt[h];
       }
       __syncthreads();
+      if (slabs != nullptr) {
+        // the integration's sums in the fused integrator's slab layout, for
+        // the adjoint's T-cotangent pre-pass (valid integrations only: it
+        // reads no other)
+        long long* sl = slabs + it * slab_stride(P, B);
+        for (int64_t e = e0 + tid; e < e1; e += BL_THREADS) {
+          int64_t o = e;
+          if (e >= P) {
+            const int64_t blk = (e - P) / B, a = (e - P) - blk * B;
+            o = slab_rz_len(P) + kApRec * a + block_comp((int)blk);
+          }
+          sl[o] = hist[e - e0];
+        }
+      }
       const double T = T_int[it];
       const double un_t = fix_unscale(e_t);
 #pragma unroll
@@ -798,6 +813,11 @@ int64_t smx_library_workspace_bytes(const smx_grids* g, int64_t n_int) {
   return slab_stride(g->nR * g->nz * g->nphi, g->num_vbin) * n_int * (int64_t)sizeof(double);
 }
 
+int64_t smx_library_ckpt_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int) {
+  if (!g || !s || n_int < 0) return -1;
+  return smx_library_workspace_bytes(g, n_int) + sample_bytes(s, n_int);
+}
+
 int64_t smx_library_build_bytes(const smx_grids* g, const smx_schedule* s, int64_t n_int) {
   if (!g || !s || n_int < 0) return -1;
   const int64_t slabs = smx_library_workspace_bytes(g, n_int);
@@ -841,8 +861,9 @@ static int library_window(int64_t rows, int64_t B, int64_t* win, int* stage, siz
 // (integrate_library_kernel<true>, then bin_library_kernel), whose workspace
 // holds the sample rows; otherwise the fused integrator with fixed-point
 // slabs in the workspace, then finalize_library_kernel.  Both give the same
-// bits.  Checkpoints (the adjoint's, whose T-cotangent pre-pass reads the
-// slabs) use the fused form.
+// bits.  With checkpoints (the adjoint's) the integrations' slabs are kept
+// for the T-cotangent pre-pass: the binning kernel writes them too, the
+// sample rows following them in the workspace.
 static int build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                          const double* wb, const double* T_total, const double* dt0, int64_t n,
                          int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
@@ -854,7 +875,6 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
               "smx_build_library: null pointer");
   SMX_REQUIRE(n >= 0 && r >= 1 && s->N_max >= 0, "smx_build_library: bad sizes");
   SMX_REQUIRE(ckpt == nullptr || ckpt_every >= 1, "smx_build_library_ckpt: ckpt_every must be >= 1");
-  SMX_REQUIRE(!(samples && ckpt), "smx_build_library: checkpoints need the fused integrator");
   if (int e = check_grids(g)) return e;
   if (n == 0) return 0;
   const int64_t n_int = n * r;
@@ -867,22 +887,26 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
     int stage = 0;
     size_t lds_bytes = 0;
     if (int e = library_window((int64_t)P + 6 * (int64_t)B, B, &win, &stage, &lds_bytes)) return e;
-    double* smp = static_cast<double*>(workspace);
+    // with checkpoints the workspace holds the slabs (kept for the adjoint)
+    // and then the sample rows
+    long long* slabs = ckpt ? static_cast<long long*>(workspace) : nullptr;
+    double* smp = ckpt ? reinterpret_cast<double*>(slabs + slab_stride(P, B) * n_int)
+                       : static_cast<double*>(workspace);
     stage_begin(ST_INTEGRATE, st);
     hipLaunchKernelGGL(integrate_library_kernel<true>, dim3((unsigned)integrator_waves<true>(n_int)),
                        dim3(64), 0, st, *p, *g, *s, wb, T_total, dt0, n_int, bin_mapping, v0, sdisp,
-                       nullptr, smp, valid, T_int, t_final, nullptr, 1);
+                       nullptr, smp, valid, T_int, t_final, ckpt, ckpt ? ckpt_every : 1);
     SMX_LAUNCHED();
     stage_end(ST_INTEGRATE, st, 1);
     stage_begin(ST_FINALIZE, st);
     if (NC == 4)
       hipLaunchKernelGGL(bin_library_kernel<4>, dim3((unsigned)n), dim3(BL_THREADS), lds_bytes, st,
                          *g, *s, smp, n, (int)r, dt0, valid, T_int, bin_mapping, v0, sdisp,
-                         num_per_bin, win, stage, lib);
+                         num_per_bin, win, stage, lib, slabs);
     else
       hipLaunchKernelGGL(bin_library_kernel<1>, dim3((unsigned)n), dim3(BL_THREADS), lds_bytes, st,
                          *g, *s, smp, n, (int)r, dt0, valid, T_int, bin_mapping, v0, sdisp,
-                         num_per_bin, win, stage, lib);
+                         num_per_bin, win, stage, lib, slabs);
     SMX_LAUNCHED();
     stage_end(ST_FINALIZE, st, 1);
     return 0;
@@ -977,7 +1001,8 @@ int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx
                            int32_t ckpt_every, void* stream) {
   SMX_REQUIRE(ckpt != nullptr, "smx_build_library_ckpt: null checkpoint buffer");
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
-                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every, false, stream);
+                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every,
+                       knob(KN_LIB_BINNING) == 1, stream);
 }
 
 int smx_build_library_traj(const smx_potential* p, const smx_grids* g, const smx_schedule* s,

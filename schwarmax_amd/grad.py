@@ -146,8 +146,8 @@ class _Library(torch.autograd.Function):
         n_int = n * r
         P = grids.nR * grids.nz * grids.nphi
         B = grids.num_vbin
-        ws = torch.empty(int(L.smx_library_workspace_bytes(grids, n_int)), dtype=torch.uint8,
-                         device='cuda')
+        ws = torch.empty(int(L.smx_library_ckpt_bytes(grids, sched_struct, n_int)),
+                         dtype=torch.uint8, device='cuda')
         lib = torch.empty((n, P + 6 * B), dtype=torch.float64, device='cuda')
         valid = torch.empty(n_int, dtype=torch.float64, device='cuda')
         T_int = torch.empty(n_int, dtype=torch.float64, device='cuda')

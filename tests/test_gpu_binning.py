@@ -7,7 +7,10 @@ register caches and finalize_library_kernel averages them.  Both add the same
 64-bit fixed-point contributions, so library, validity flags, T and t_final
 must be identical, with and without the fourfold symmetry, and for a grid
 whose rows (cells + 6 x apertures) exceed one LDS window of 20,480, which
-bin_library_kernel covers in two passes.  Each case also checks that the
+bin_library_kernel covers in two passes.  The checkpointed build for the
+gradient (the binning kernel also writes each integration's slab for the
+adjoint's T-cotangent pre-pass) gives the same logL and gradient bit for bit
+in both forms.  Each case also checks that the
 library is not trivial (most integrations valid, non-zero surface density and
 h terms).  The default form against the FP64 oracle (integrates.py:115-238):
 test_gpu_stages.py::test_library_matches_oracle and every library parity test.
@@ -93,3 +96,27 @@ def test_sample_binning_two_windows():
     g = args[1]
     assert g.nR * g.nz * g.nphi + 6 * g.num_vbin > 20480
     _compare(args)
+
+
+def test_gradient_equal_in_both_forms():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from schwarmax_amd import grad as GR, tune
+    from galaxies import observed_galaxy
+    theta, dd, sched = observed_galaxy(seed=11, n=40, n_ap=24, N_max=500)[:3]
+    before = tune.get_knob(tune.KNOB_LIB_BINNING)
+    out = {}
+    try:
+        for form in (1, 2):
+            tune.set_knob(tune.KNOB_LIB_BINNING, form)
+            print(f"gradient, binning form {form}", flush=True)
+            v, g = GR.model_likelihood_and_grad(theta, dd, 24, dd['Rzphi_n_tot'], schedule=sched,
+                                                nnls_maxiter=30, ckpt_every=16)
+            out[form] = (float(v), g.numpy())
+    finally:
+        tune.set_knob(tune.KNOB_LIB_BINNING, before)
+    assert np.isfinite(out[1][0]) and np.all(np.isfinite(out[1][1]))
+    assert np.any(out[1][1] != 0.0)
+    assert out[1][0] == out[2][0]
+    assert np.array_equal(out[1][1], out[2][1])

@@ -69,8 +69,9 @@ int smx_last_error(char* buf, int64_t len);
 int smx_version(void);
 /* Per-stage timing with HIP events on the launch stream (no counterpart in
  * the reference; used by the benchmark).  Stages, in order: initial
- * conditions, integration, finalize, Gram, factor + inverse, right-hand sides,
- * ADMM loop, model maps, density target, and the ADMM product kernel alone
+ * conditions, integration, finalize (the sample binning and realization
+ * average, or the fused form's finalize), Gram, factor + inverse, right-hand
+ * sides, ADMM loop, model maps, density target, and the ADMM product kernel alone
  * (one record per launch, nested inside the ADMM loop).  enable != 0 turns recording on and
  * clears previous records; read fills ms[] and launches[] (kernel launches per
  * stage) summed over the records since, then clears them. */

@@ -44,6 +44,30 @@ def test_split_gemm_refuses_bad_arguments_without_gpu():
     assert L.smx_gemm_nt_f64_split(4, 8, 16, 1.0, fake, 15, fake, 16, fake, 8, fake, 64, None) == 2
 
 
+def test_loglike_entry_points_refuse_bad_arguments_without_gpu():
+    """smx_model_epilogue and smx_bootstrap_loglike check pointers and sizes
+    before touching the device."""
+    from schwarmax_amd import _lib
+    L = _lib.load_library()
+    fake = ctypes.c_void_p(256)                     # never dereferenced: refused first
+    assert L.smx_model_epilogue(fake, 10, 2, 4, 3, None, fake, fake, 1.0, 1.0, fake, fake, fake,
+                                fake, None) == 2
+    assert 'smx_model_epilogue' in _lib.last_error()
+    # leading dimension below P + 5B
+    assert L.smx_model_epilogue(fake, 18, 2, 4, 3, fake, fake, fake, 1.0, 1.0, fake, fake, fake,
+                                fake, None) == 2
+    model, data = _lib.SmxBootstrapModel(), _lib.SmxBootstrapData()
+    assert L.smx_bootstrap_loglike(model, 4, 5, 3, data, 1.0, fake, fake, fake, None) == 2
+    assert 'null model map' in _lib.last_error()
+    model.d3 = model.y_R = model.surface_density = 256
+    data.xy = data.xy_err = data.xy_normal = 256
+    for j in range(4):
+        model.h[j] = data.h[j] = data.h_err[j] = data.h_normal[j] = 256
+    model.ld3 = 4                                   # below P = 5
+    assert L.smx_bootstrap_loglike(model, 4, 5, 3, data, 1.0, fake, fake, fake, None) == 2
+    assert 'bad sizes' in _lib.last_error()
+
+
 def test_grouped_nnls_refuses_bad_arguments_without_gpu():
     """smx_nnls_admm_group checks its arguments before touching the device."""
     from schwarmax_amd import _lib
@@ -76,6 +100,9 @@ def test_workspace_queries_without_gpu():
         sc = _lib.SmxSchedule(N_max=N_max, Omega=0.0, atol=1e-6, rtol=1e-4, dt_min=1e-5, dt_max=0.3)
         assert L.smx_library_build_bytes(g, sc, 10) == max(slabs, 10 * N_max * 64)
     assert L.smx_library_build_bytes(g, sc, -1) == -1
+    # the checkpointed build: the slabs (which the adjoint reads), then the rows
+    assert L.smx_library_ckpt_bytes(g, sc, 10) == slabs + 10 * 5000 * 64
+    assert L.smx_library_ckpt_bytes(g, sc, -1) == -1
     assert L.smx_nnls_workspace_bytes(1000, 700, 20) > 4 * 1000 * 1000
     assert L.smx_nnls_workspace_bytes(0, 1, 1) == -1
 

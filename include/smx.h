@@ -121,14 +121,16 @@ int64_t smx_library_build_bytes(const smx_grids* g, const smx_schedule* s, int64
  * (SMX_KNOB_LIB_BINNING = 2 selects the fused integrator, which bins into
  * per-integration slabs; both give the same bits: the sums are 64-bit fixed
  * point).  valid/T_int/t_final (n_int,) are per-integration diagnostics.
- * workspace: smx_library_build_bytes(g, s, n*r) bytes.
+ * workspace: workspace_bytes bytes; the sample binning runs when they are at
+ * least smx_library_build_bytes(g, s, n*r), the fused form when they only hold
+ * the slabs (smx_library_workspace_bytes); fewer are refused (code 2).
  * Replaces _integrate_adaptive_batch_chunked_vmap (integrates.py:303-313). */
 int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                       const double* wb, const double* T_total, const double* dt0,
                       int64_t n, int32_t r, const int32_t* bin_mapping,
                       const double* num_per_bin, const double* v0, const double* sdisp,
-                      void* workspace, double* lib, double* valid, double* T_int,
-                      double* t_final, void* stream);
+                      void* workspace, int64_t workspace_bytes, double* lib, double* valid,
+                      double* T_int, double* t_final, void* stream);
 /* No counterpart in the reference: smx_build_library for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments and results as
@@ -199,14 +201,16 @@ int smx_density_target(const smx_potential* p, const double* R_grid, const doubl
  * ckpt[(j, i, 0:8)] = (y[6], dt, t) of integration i before attempt
  * j * ckpt_every, j < ceil(N_max / ckpt_every), and keeps each valid
  * integration's fixed-point sums in the slabs at the start of `workspace`
- * (smx_library_ckpt_bytes; the sample rows follow them).  The slabs stay
+ * (smx_library_ckpt_bytes bytes for the sample binning, the rows following the
+ * slabs; with only the slabs' bytes the fused form runs).  The slabs stay
  * valid until the caller reuses the workspace; smx_library_vjp reads them. */
 int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                            const double* wb, const double* T_total, const double* dt0,
                            int64_t n, int32_t r, const int32_t* bin_mapping,
                            const double* num_per_bin, const double* v0, const double* sdisp,
-                           void* workspace, double* lib, double* valid, double* T_int,
-                           double* t_final, double* ckpt, int32_t ckpt_every, void* stream);
+                           void* workspace, int64_t workspace_bytes, double* lib, double* valid,
+                           double* T_int, double* t_final, double* ckpt, int32_t ckpt_every,
+                           void* stream);
 /* Scratch for smx_library_vjp: per integration ckpt_every * 31 doubles of
  * replayed attempts plus two scalars. */
 int64_t smx_library_vjp_workspace_bytes(int64_t n_int, int32_t ckpt_every);

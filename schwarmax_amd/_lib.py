@@ -103,13 +103,13 @@ _SIGS = [
      [ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule), _I64]),
     ('smx_build_library', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
-      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]),
     ('smx_build_library_noalloc', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
       _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ('smx_build_library_ckpt', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),
-      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _P]),
+      _P, _P, _P, _I64, _I32, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _I32, _P]),
     ('smx_library_vjp_workspace_bytes', _I64, [_I64, _I32]),
     ('smx_library_vjp', ctypes.c_int,
      [ctypes.POINTER(SmxPotential), ctypes.POINTER(SmxGrids), ctypes.POINTER(SmxSchedule),

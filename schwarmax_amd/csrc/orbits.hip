@@ -867,9 +867,10 @@ static int library_window(int64_t rows, int64_t B, int64_t* win, int* stage, siz
 static int build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                          const double* wb, const double* T_total, const double* dt0, int64_t n,
                          int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
-                         const double* v0, const double* sdisp, void* workspace, double* lib,
-                         double* valid, double* T_int, double* t_final, double* ckpt,
-                         int32_t ckpt_every, bool samples, void* stream) {
+                         const double* v0, const double* sdisp, void* workspace,
+                         int64_t workspace_bytes, double* lib, double* valid, double* T_int,
+                         double* t_final, double* ckpt, int32_t ckpt_every, bool samples,
+                         void* stream) {
   SMX_REQUIRE(p && g && s && wb && T_total && dt0 && bin_mapping && num_per_bin && v0 && sdisp &&
                   workspace && lib && valid && T_int && t_final,
               "smx_build_library: null pointer");
@@ -881,6 +882,15 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
   const int P = g->nR * g->nz * g->nphi;
   const int B = g->num_vbin;
   const int NC = g->symmetry ? 4 : 1;
+  // the sample binning when the workspace holds its rows (after the slabs
+  // with checkpoints), else the fused form when it holds the slabs
+  const int64_t slab_need = smx_library_workspace_bytes(g, n_int);
+  const int64_t smp_need = (ckpt ? slab_need : 0) + sample_bytes(s, n_int);
+  samples = samples && workspace_bytes >= smp_need;
+  SMX_REQUIRE(samples || workspace_bytes >= slab_need,
+              "smx_build_library: workspace of %lld bytes, the fused form needs %lld and the "
+              "sample binning %lld", (long long)workspace_bytes, (long long)slab_need,
+              (long long)smp_need);
   hipStream_t st = as_stream(stream);
   if (samples) {
     int64_t win = 0;
@@ -935,10 +945,11 @@ static int build_library(const smx_potential* p, const smx_grids* g, const smx_s
 int smx_build_library(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                       const double* wb, const double* T_total, const double* dt0, int64_t n,
                       int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
-                      const double* v0, const double* sdisp, void* workspace, double* lib,
-                      double* valid, double* T_int, double* t_final, void* stream) {
+                      const double* v0, const double* sdisp, void* workspace,
+                      int64_t workspace_bytes, double* lib, double* valid, double* T_int,
+                      double* t_final, void* stream) {
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
-                       workspace, lib, valid, T_int, t_final, nullptr, 0,
+                       workspace, workspace_bytes, lib, valid, T_int, t_final, nullptr, 0,
                        knob(KN_LIB_BINNING) == 1, stream);
 }
 
@@ -990,18 +1001,18 @@ int smx_build_library_noalloc(const smx_potential* p, const smx_grids* g, const 
   double* ws = nullptr;
   if (int e = scratch_for_current_device(&ws)) return e;
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp, ws,
-                       lib, valid, T_int, t_final, nullptr, 0, false, stream);
+                       SMX_NOALLOC_SCRATCH_BYTES, lib, valid, T_int, t_final, nullptr, 0, false, stream);
 }
 
 int smx_build_library_ckpt(const smx_potential* p, const smx_grids* g, const smx_schedule* s,
                            const double* wb, const double* T_total, const double* dt0, int64_t n,
                            int32_t r, const int32_t* bin_mapping, const double* num_per_bin,
-                           const double* v0, const double* sdisp, void* workspace, double* lib,
-                           double* valid, double* T_int, double* t_final, double* ckpt,
-                           int32_t ckpt_every, void* stream) {
+                           const double* v0, const double* sdisp, void* workspace,
+                           int64_t workspace_bytes, double* lib, double* valid, double* T_int,
+                           double* t_final, double* ckpt, int32_t ckpt_every, void* stream) {
   SMX_REQUIRE(ckpt != nullptr, "smx_build_library_ckpt: null checkpoint buffer");
   return build_library(p, g, s, wb, T_total, dt0, n, r, bin_mapping, num_per_bin, v0, sdisp,
-                       workspace, lib, valid, T_int, t_final, ckpt, ckpt_every,
+                       workspace, workspace_bytes, lib, valid, T_int, t_final, ckpt, ckpt_every,
                        knob(KN_LIB_BINNING) == 1, stream);
 }
 

@@ -162,7 +162,8 @@ class _Library(torch.autograd.Function):
         dt0 = dt0.contiguous()
         _lib.check(L.smx_build_library_ckpt(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),
                                             _lib.ptr(dt0), n, r, _lib.ptr(bm), _lib.ptr(npb),
-                                            _lib.ptr(v0), _lib.ptr(sd), _lib.ptr(ws), _lib.ptr(lib),
+                                            _lib.ptr(v0), _lib.ptr(sd), _lib.ptr(ws), ws.numel(),
+                                            _lib.ptr(lib),
                                             _lib.ptr(valid), _lib.ptr(T_int), _lib.ptr(t_final),
                                             _lib.ptr(ckpt), int(ckpt_every), _lib.stream_ptr()),
                    'smx_build_library_ckpt')

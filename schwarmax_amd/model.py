@@ -159,8 +159,9 @@ def build_library(pot, grids, sched_struct, wb, Tt, dt0, dict_data, traj=False):
         t_final = torch.empty(n_int, dtype=torch.float64, device='cuda')
         _lib.check(L.smx_build_library(pot, grids, sched_struct, _lib.ptr(wb), _lib.ptr(Tt),
                                        _lib.ptr(dt0), n, r, _lib.ptr(bm), _lib.ptr(npb), _lib.ptr(v0),
-                                       _lib.ptr(sd), _lib.ptr(ws), _lib.ptr(lib), _lib.ptr(valid),
-                                       _lib.ptr(T_int), _lib.ptr(t_final), _lib.stream_ptr()),
+                                       _lib.ptr(sd), _lib.ptr(ws), ws.numel(), _lib.ptr(lib),
+                                       _lib.ptr(valid), _lib.ptr(T_int), _lib.ptr(t_final),
+                                       _lib.stream_ptr()),
                    'smx_build_library')
         extra = {'t_final': t_final}
     del ws

@@ -68,6 +68,25 @@ def test_loglike_entry_points_refuse_bad_arguments_without_gpu():
     assert 'bad sizes' in _lib.last_error()
 
 
+def test_library_build_refuses_small_workspace_without_gpu():
+    """smx_build_library and smx_build_library_ckpt refuse, before any launch,
+    a workspace that holds neither the sample rows nor the fused form's slabs."""
+    from schwarmax_amd import _lib, model as M
+    L = _lib.load_library()
+    g = M.make_grids(100, (8, 4, 6), [[0, 10], [-3, 3], [-np.pi, np.pi]], [[-5, 5], [-5, 5]],
+                     (20, 20), np.eye(3), True)
+    sc = _lib.SmxSchedule(N_max=50, Omega=0.0, atol=1e-6, rtol=1e-4, dt_min=1e-5, dt_max=0.3)
+    pot = _lib.SmxPotential()
+    fake = ctypes.c_void_p(256)                     # never dereferenced: refused first
+    small = min(L.smx_library_workspace_bytes(g, 10 * 4), 10 * 4 * 50 * 64) - 8
+    args = (pot, g, sc, fake, fake, fake, 10, 4, fake, fake, fake, fake, fake, small, fake, fake,
+            fake, fake)
+    assert L.smx_build_library(*args, None) == 2
+    assert 'workspace of' in _lib.last_error()
+    assert L.smx_build_library_ckpt(*args, fake, 16, None) == 2
+    assert 'workspace of' in _lib.last_error()
+
+
 def test_grouped_nnls_refuses_bad_arguments_without_gpu():
     """smx_nnls_admm_group checks its arguments before touching the device."""
     from schwarmax_amd import _lib

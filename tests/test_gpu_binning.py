@@ -57,7 +57,8 @@ def _build(args, form):
         _lib.check(L.smx_build_library(pot, g, sc, _lib.ptr(wb, f64), _lib.ptr(Tt, f64),
                                        _lib.ptr(dt0, f64), n, r, _lib.ptr(bm, torch.int32),
                                        _lib.ptr(npb, f64), _lib.ptr(v0, f64), _lib.ptr(sd, f64),
-                                       _lib.ptr(ws), _lib.ptr(lib, f64), _lib.ptr(valid, f64),
+                                       _lib.ptr(ws), ws.numel(), _lib.ptr(lib, f64),
+                                       _lib.ptr(valid, f64),
                                        _lib.ptr(T_int, f64), _lib.ptr(t_final, f64),
                                        _lib.stream_ptr()), 'smx_build_library')
         torch.cuda.synchronize()

@@ -56,7 +56,7 @@ def _run(noalloc, args):
     else:
         ws = torch.empty(int(L.smx_library_build_bytes(g, sc, n * r)), dtype=torch.uint8,
                          device='cuda')
-        _lib.check(L.smx_build_library(*common, _lib.ptr(ws), *outs, _lib.stream_ptr()),
+        _lib.check(L.smx_build_library(*common, _lib.ptr(ws), ws.numel(), *outs, _lib.stream_ptr()),
                    'smx_build_library')
     torch.cuda.synchronize()
     return [t.cpu().numpy() for t in (lib, valid, T_int, t_final)]

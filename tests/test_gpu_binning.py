@@ -99,13 +99,16 @@ def test_sample_binning_two_windows():
     _compare(args)
 
 
-def test_gradient_equal_in_both_forms():
+@pytest.mark.parametrize("grid,n", [((8, 4, 6), 40), ((40, 32, 20), 16)])
+def test_gradient_equal_in_both_forms(grid, n):
+    """One LDS window, and two (25,600 cells): the slabs the binning kernel
+    writes for the adjoint equal the fused integrator's."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from schwarmax_amd import grad as GR, tune
     from galaxies import observed_galaxy
-    theta, dd, sched = observed_galaxy(seed=11, n=40, n_ap=24, N_max=500)[:3]
+    theta, dd, sched = observed_galaxy(seed=11, n=n, n_ap=24, grid=grid, N_max=500)[:3]
     before = tune.get_knob(tune.KNOB_LIB_BINNING)
     out = {}
     try:
